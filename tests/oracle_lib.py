@@ -1,0 +1,266 @@
+"""ctypes access to the FP64 oracle (oracle/liboracle.so) and, where it was
+built in this container, to the reference's own sources (oracle/_ref/libref.so).
+
+Test infrastructure only: the product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_L = C.c_long
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _build_oracle():
+    so = os.path.join(ORACLE_DIR, "liboracle.so")
+    src = os.path.join(ORACLE_DIR, "mm_oracle.c")
+    if not os.path.exists(so) or os.path.getmtime(so) < os.path.getmtime(src):
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR, "all"], check=True)
+    return so
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        lib = C.CDLL(_build_oracle())
+        lib.orc_nl_build.restype = _L
+        lib.orc_nl_build.argtypes = [_I, _P, _D, _D, _D, _P, _L]
+        lib.orc_nl_ncell.restype = _I
+        lib.orc_nl_nmax.restype = _I
+        lib.orc_nl_nmax.argtypes = [_I, _D, _D, _D]
+        lib.orc_nl_cell_pairs.argtypes = [_I, _D, _D, _D, _P, _P, _I]
+        lib.orc_nl_needs_rebuild.argtypes = [_I, _P, _P, _D]
+        lib.orc_cheby_rspace.argtypes = [_P, _D, _D]
+        lib.orc_cheby_eval.argtypes = [_P, _D, _P, _P]
+        lib.orc_rspace_ewald.argtypes = [_I, _I, _P, _P, _D, _L, _P, _I, _P, _D, _D, _P, _P,
+                                         _D, _P, _P, _P, _P, _I, _I, _P, _D, _D,
+                                         _P, _P, _P, _P, _P]
+        lib.orc_kspace_count.argtypes = [_D]
+        lib.orc_kspace_vectors.argtypes = [_D, _P]
+        lib.orc_kspace_field.argtypes = [_D, _I, _P, _P, _P, _D, _D, _P, _P, _P, _P]
+        lib.orc_p3m_grid_size.argtypes = [_D, _D]
+        lib.orc_p3m_green.restype = _D
+        lib.orc_p3m_green.argtypes = [_I, _I, _I, _P, _D, _I, _I, _I, _P]
+        lib.orc_p3m_weights.argtypes = [_D, _I, _P, _P]
+        lib.orc_p3m_field.argtypes = [_I, _P, _P, _I, _I, _I, _P, _D, _I, _I, _P, _P, _P, _P, _P]
+        lib.orc_fft3d.argtypes = [_I, _I, _I, _P, _P, _I]
+        lib.orc_self_phi.argtypes = [_I, _P, _D, _D, _P]
+        lib.orc_lj_tail.restype = _D
+        lib.orc_lj_tail.argtypes = [_I, _P, _P, _P, _I, _D, _D, _D]
+        lib.orc_evaluate.argtypes = [_P, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]
+        lib.orc_md_run.argtypes = [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _D, _I,
+                                   _D, _D, _D, _P]
+        lib.orc_constrain_positions.argtypes = [_I, _P, _P, _P, _P, _D, _D, _D, _D, _I]
+        lib.orc_constrain_velocities.argtypes = [_I, _P, _P, _P, _D, _D]
+        lib.orc_pbin_build.argtypes = [_I, _I, _P, _P, _P]
+        lib.orc_pbin_exists.argtypes = [_P, _P, _I, _I]
+        _oracle = lib
+    return _oracle
+
+
+def ref():
+    """The compiled reference harness, or None when it was not built here."""
+    global _ref
+    if _ref is None:
+        so = os.path.join(ORACLE_DIR, "_ref", "libref.so")
+        if not os.path.exists(so):
+            if os.path.isdir("/root/reference/src"):
+                subprocess.run(["make", "-s", "-C", ORACLE_DIR, "ref"], check=True,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            if not os.path.exists(so):
+                return None
+        lib = C.CDLL(so)
+        lib.ref_nl_pairs.restype = _L
+        lib.ref_nl_pairs.argtypes = [_I, _P, _D, _D, _D, _P, _L]
+        lib.ref_nl_pairs_two_step.restype = _L
+        lib.ref_nl_pairs_two_step.argtypes = [_I, _P, _P, _D, _D, _D, _P, _L]
+        lib.ref_cheby.argtypes = [_D, _D, _P]
+        lib.ref_rspace.argtypes = [_I, _I, _P, _P, _D, _D, _D, _D, _I, _P, _P, _P, _D, _P, _P,
+                                   _P, _P, _P, _P]
+        lib.ref_kspace.argtypes = [_D, _I, _P, _P, _P, _D, _D, _P, _P]
+        lib.ref_kspace_count.argtypes = [_D]
+        lib.ref_pbin_kat.argtypes = [_P, _I, _I, _P, _P, _P]
+        _ref = lib
+    return _ref
+
+
+# ---- convenience wrappers ------------------------------------------------
+
+class OrcParams(C.Structure):
+    _fields_ = [("boxl", _D), ("rc", _D), ("smoothing_width", _D), ("skin", _D), ("eta", _D),
+                ("kmode", _I), ("kc", _D), ("nx", _I), ("ny", _I), ("nz", _I), ("order", _I),
+                ("nalias", _I), ("ik", _I), ("include_lj_correction", _I),
+                ("geometric_sigma", _I)]
+
+
+def params(system, p):
+    mesh = p.get("mesh", (0, 0, 0))
+    return OrcParams(system.boxl, p["rc"], p["smoothing_width"], p["skin"], p["eta"],
+                     p["kmode"], p.get("kc", 0.0), mesh[0], mesh[1], mesh[2], p.get("order", 5),
+                     p.get("nalias", 2), p.get("ik", 0), p.get("include_lj_correction", 1), 0)
+
+
+def first_sites(system, r=None):
+    r = system.r if r is None else r
+    return np.ascontiguousarray(r[system.group[:, 0]])
+
+
+def nl_pairs(gr, boxl, rc, skin):
+    lib = oracle()
+    gr = np.ascontiguousarray(gr, dtype=np.float64)
+    n = gr.shape[0]
+    npair = lib.orc_nl_build(n, _ptr(gr), boxl, rc, skin, None, 0)
+    out = np.zeros((max(npair, 1), 2), dtype=np.int32)
+    lib.orc_nl_build(n, _ptr(gr), boxl, rc, skin, _ptr(out), npair)
+    return out[:npair]
+
+
+def ref_nl_pairs(gr, boxl, rc, skin):
+    lib = ref()
+    gr = np.ascontiguousarray(gr, dtype=np.float64)
+    n = gr.shape[0]
+    npair = lib.ref_nl_pairs(n, _ptr(gr), boxl, rc, skin, None, 0)
+    out = np.zeros((max(npair, 1), 2), dtype=np.int32)
+    lib.ref_nl_pairs(n, _ptr(gr), boxl, rc, skin, _ptr(out), npair)
+    return out[:npair]
+
+
+def pair_set(pairs):
+    """Canonical (row, col) pairs as a sorted structured key array."""
+    p = np.asarray(pairs, dtype=np.int64)
+    return np.unique(p[:, 0] * (1 << 32) + p[:, 1])
+
+
+def evaluate(system, p, r=None, virial=False):
+    lib = oracle()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    f = np.zeros_like(r)
+    en = np.zeros(4)
+    vir = np.zeros(9) if virial else None
+    par = params(system, p)
+    rc = lib.orc_evaluate(C.byref(par), system.nsite, system.ngroup,
+                          _ptr(np.ascontiguousarray(system.group)), _ptr(r),
+                          _ptr(system.q), _ptr(system.sig), _ptr(system.eps),
+                          _ptr(system.site_mask), len(system.excl),
+                          _ptr(np.ascontiguousarray(system.excl)), _ptr(f), _ptr(en), _ptr(vir))
+    assert rc == 0
+    return (f, en, vir) if virial else (f, en)
+
+
+def rspace_parts(system, p, r=None):
+    """phi, evec, lj_force, lj_energy of RSpaceEwald alone (oracle)."""
+    lib = oracle()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    gr = first_sites(system, r)
+    pairs = nl_pairs(gr, system.boxl, p["rc"], p["skin"])
+    order = np.argsort(pairs[:, 0], kind="stable")
+    nl = np.ascontiguousarray(pairs[order])
+    cheb = (C.c_double * (3 + 512 * 7))()
+    lib.orc_cheby_rspace(cheb, p["rc"], p["eta"])
+    n = system.nsite
+    phi = np.zeros(n); ev = np.zeros((n, 3)); ljf = np.zeros((n, 3)); lje = np.zeros(1)
+    lib.orc_rspace_ewald(n, system.ngroup, _ptr(np.ascontiguousarray(system.group)), _ptr(r),
+                         system.boxl, len(nl), _ptr(nl), len(system.excl),
+                         _ptr(np.ascontiguousarray(system.excl)),
+                         (p["rc"] - p["smoothing_width"]) ** 2, p["rc"] ** 2,
+                         _ptr(system.site_mask), _ptr(system.q), p["eta"], cheb, None,
+                         _ptr(system.sig), _ptr(system.eps), 0, 0, None, 1.0, 1.0,
+                         _ptr(phi), _ptr(ev), _ptr(lje), _ptr(ljf), None)
+    return phi, ev, ljf, lje[0]
+
+
+def ref_rspace_parts(system, p, r=None):
+    lib = ref()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    n = system.nsite
+    phi = np.zeros(n); ev = np.zeros((n, 3)); ljf = np.zeros((n, 3)); lje = np.zeros(1)
+    lib.ref_rspace(n, system.ngroup, _ptr(np.ascontiguousarray(system.group)), _ptr(r),
+                   system.boxl, p["rc"], p["smoothing_width"], p["skin"], len(system.excl),
+                   _ptr(np.ascontiguousarray(system.excl)), _ptr(system.site_mask),
+                   _ptr(system.q), p["eta"], _ptr(system.sig), _ptr(system.eps),
+                   _ptr(phi), _ptr(ev), _ptr(lje), _ptr(ljf))
+    return phi, ev, ljf, lje[0]
+
+
+def recip_cubic(boxl):
+    """2 pi (L^-1)^T for a cubic box, tensor_t layout."""
+    a = 2 * np.pi * (1.0 / boxl)
+    return np.array([a, 0, 0, 0, a, 0, 0, 0, a], dtype=np.float64)
+
+
+def kspace_parts(system, kc, eta, r=None):
+    lib = oracle()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    n = system.nsite
+    phi = np.zeros(n); ev = np.zeros((n, 3)); vir = np.zeros(9); en = np.zeros(1)
+    L = system.boxl
+    inv = np.array([1.0 / L, 0, 0, 0, 1.0 / L, 0, 0, 0, 1.0 / L])
+    rl = 2 * np.pi * inv
+    lib.orc_kspace_field(kc, n, _ptr(r), _ptr(system.q), _ptr(rl), L ** 3, eta,
+                         _ptr(phi), _ptr(ev), _ptr(vir), _ptr(en))
+    return phi, ev, vir, en[0]
+
+
+def ref_kspace_parts(system, kc, eta, r=None):
+    lib = ref()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    n = system.nsite
+    phi = np.zeros(n); ev = np.zeros((n, 3))
+    L = system.boxl
+    rl = 2 * np.pi * np.array([1.0 / L, 0, 0, 0, 1.0 / L, 0, 0, 0, 1.0 / L])
+    lib.ref_kspace(kc, n, _ptr(r), _ptr(system.q), _ptr(rl), L ** 3, eta, _ptr(phi), _ptr(ev))
+    return phi, ev
+
+
+def p3m_parts(system, p, r=None, virial=False):
+    lib = oracle()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    nx, ny, nz = p["mesh"]
+    L = system.boxl
+    latv = np.array([L, 0, 0, 0, L, 0, 0, 0, L], dtype=np.float64)
+    gh = np.zeros(nx * ny * (nz // 2 + 1))
+    lib.orc_p3m_green(nx, ny, nz, _ptr(latv), p["eta"], p.get("nalias", 2), p["order"],
+                      p.get("ik", 0), _ptr(gh))
+    n = system.nsite
+    phi = np.zeros(n); ev = np.zeros((n, 3)); vir = np.zeros(9); en = np.zeros(1)
+    lib.orc_p3m_field(n, _ptr(r), _ptr(system.q), nx, ny, nz, _ptr(latv), p["eta"], p["order"],
+                      p.get("ik", 0), _ptr(gh), _ptr(phi), _ptr(ev), _ptr(vir), _ptr(en))
+    return phi, ev, vir, en[0]
+
+
+def green(mesh, boxl, eta, order, nalias=2, ik=0):
+    lib = oracle()
+    nx, ny, nz = mesh
+    latv = np.array([boxl, 0, 0, 0, boxl, 0, 0, 0, boxl], dtype=np.float64)
+    gh = np.zeros(nx * ny * (nz // 2 + 1))
+    chi = lib.orc_p3m_green(nx, ny, nz, _ptr(latv), eta, nalias, order, ik, _ptr(gh))
+    return gh.reshape(nx, ny, nz // 2 + 1), chi
+
+
+def cheby(rc, eta):
+    lib = oracle()
+    t = np.zeros(3 + 512 * 7)
+    lib.orc_cheby_rspace(_ptr(t), rc, eta)
+    return t
+
+
+def ref_cheby(rc, eta):
+    t = np.zeros(3 + 512 * 7)
+    ref().ref_cheby(rc, eta, _ptr(t))
+    return t
