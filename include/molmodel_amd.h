@@ -1,0 +1,145 @@
+/*
+ * molmodel_amd.h -- C ABI of the MI355X nonbonded force path.
+ *
+ * Two layers:
+ *  1. Device-resident engine (mm_*): a batch of periodic cubic systems whose
+ *     positions, velocities and forces stay in HBM.  One call evaluates the
+ *     full ElecLJ force (neighbor list, real-space Ewald + LJ, plain Ewald
+ *     k-space or P3M, self/background terms) or runs velocity-Verlet MD with
+ *     rigid-water constraints.  This is what replaces
+ *     ElecLJ::add_to_energy_and_forces (eleclj.cpp:785-807) and the MD loop
+ *     body of MDyn::run (mdyn.cpp:142-160).
+ *  2. Reference-named drop-ins (RSpaceEwald, kspace_*, p3m_*, PairBinary_*,
+ *     RSpaceChebyshev_new) with the reference's signatures and accumulate
+ *     semantics (rspace.h, kspace.h, pppm.h, pbin.h, cheby.h), so existing
+ *     callers relink against this library unchanged.  Declared in
+ *     molmodel_amd_ref.h.
+ *
+ * Units are the reference's: A, kcal/mol, ps; charges in sqrt(kcal A/mol)
+ * (units.h:91-94); masses in (kcal/mol) ps^2/A^2 (units.h:166-169).
+ * Arrays of vectors are AoS double[3*n] in the caller's site order.
+ * Every function returns 0 on success or a negative MM_ERR_* code; the text
+ * of the last error is available from mm_last_error().  The reference has
+ * no return codes (it asserts or calls die(), util.c:11-26); the conditions
+ * that are fatal there are errors here.
+ * Thread safety: one engine per host thread at a time; each engine owns one
+ * HIP stream.
+ */
+#ifndef MOLMODEL_AMD_H
+#define MOLMODEL_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_OK 0
+#define MM_ERR_ARG -1        /* invalid argument / unsupported input */
+#define MM_ERR_HIP -2        /* HIP runtime failure */
+#define MM_ERR_DEVICE -3     /* device-side error flag raised (see mm_errors) */
+
+/* device error flags reported by mm_errors() */
+#define MM_FLAG_NL_OVERFLOW 1
+#define MM_FLAG_CONSTRAINT 4
+#define MM_FLAG_NAN 8
+
+#define MM_SITE_HAS_Q 1      /* rspace.h:7 */
+#define MM_SITE_HAS_LJ 2     /* rspace.h:8 */
+
+/* One periodic cubic system.  Mirrors the inputs ElecLJ::init and
+ * ElecLJ::init_sites derive (eleclj.cpp:140-265, 643-771):
+ * groups = heavy atom first, then its terminal atoms, contiguous sites;
+ * excluded pairs = 1-2 and 1-3 site pairs (must lie inside one group). */
+typedef struct {
+  int nsite, ngroup;
+  const int *group;          /* 2*ngroup: [start, end) site ranges */
+  const double *q;           /* nsite charges, sqrt(kcal A/mol) */
+  const double *sigma;       /* nsite LJ sigma (A) */
+  const double *epsilon;     /* nsite LJ epsilon (kcal/mol) */
+  const int *site_mask;      /* nsite: MM_SITE_HAS_Q | MM_SITE_HAS_LJ */
+  const double *mass;        /* nsite masses (may be NULL: no dynamics) */
+  int nexcl;
+  const int *excl;           /* 2*nexcl excluded site pairs */
+  double boxl;               /* cubic box length */
+  double rc;                 /* rspace_cutoff */
+  double smoothing_width;    /* smoothing_width (reference default 1 A) */
+  double skin;               /* neighbor_list_width */
+  double eta;                /* ewald_screening */
+  int kmode;                 /* 0: plain Ewald (kc), 1: P3M */
+  double kc;                 /* kspace_cutoff */
+  int mesh[3];               /* P3M mesh */
+  int order;                 /* p3m_assignment_order (1..7) */
+  int nalias;                /* p3m_alias_sum_limit */
+  int ik;                    /* p3m_ik_differentiate (0 only on the device) */
+  int include_lj_correction;
+  int geometric_sigma;       /* geometric_combining_for_sigma */
+} mm_system_desc;
+
+typedef struct mm_engine mm_engine;
+
+/* energy slots returned per system by mm_get_energies */
+#define MM_E_ELEC 0      /* 1/2 sum q phi */
+#define MM_E_LJ 1        /* pair LJ + tail correction */
+#define MM_E_TAIL 2      /* LJ tail correction alone */
+#define MM_E_TOTAL 3     /* ELEC + LJ */
+#define MM_E_KINETIC 4   /* kinetic energy (after mm_md_run) */
+#define MM_E_KSPACE 5    /* reciprocal-space energy of the k sum (diagnostic) */
+#define MM_NE 8
+
+/* compute flags */
+#define MM_VIRIAL 1      /* also form the 3x3 virial sum r_ij (x) f_ij */
+
+const char *mm_last_error(void);
+int mm_device_count(void);
+
+int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out);
+int mm_engine_destroy(mm_engine *e);
+/* totals over all systems of the batch */
+int mm_engine_sizes(const mm_engine *e, int *nsite, int *ngroup);
+
+/* positions / velocities, AoS double[3*nsite_total] (systems concatenated) */
+int mm_set_positions(mm_engine *e, const double *r);
+int mm_get_positions(mm_engine *e, double *r);
+int mm_set_velocities(mm_engine *e, const double *v);
+int mm_get_velocities(mm_engine *e, double *v);
+
+/* one full force evaluation at the current positions:
+ * ElecLJ::add_to_energy_and_forces (eleclj.cpp:785-807) */
+int mm_compute(mm_engine *e, int flags);
+int mm_get_forces(mm_engine *e, double *f);               /* AoS 3*nsite_total */
+int mm_get_potential(mm_engine *e, double *phi);          /* nsite_total */
+int mm_get_energies(mm_engine *e, double *en);            /* MM_NE per system */
+int mm_get_virial(mm_engine *e, double *w);               /* 9 per system, row-major */
+int mm_errors(mm_engine *e, int *flags);                  /* reads and clears */
+
+/* RSpaceEwald's outputs alone (rspace.c:382-558): phi, evec, lj_force (AoS)
+ * and lj_energy per system, including the excluded-pair correction loop;
+ * accumulated (+=) into the caller's arrays like the reference. */
+int mm_rspace_field(mm_engine *e, double *phi, double *evec, double *lj_force,
+                    double *lj_energy);
+/* reciprocal-space phi/evec alone (kspace_field or p3m_field), += */
+int mm_kspace_field(mm_engine *e, double *phi, double *evec);
+
+/* neighbor list of system `sys` as built for the current positions:
+ * canonical (row, col) group pairs (pbin.c:6) in that system's group
+ * numbering; returns the pair count (writes at most maxp pairs). */
+long mm_nl_pairs(mm_engine *e, int sys, int *pairs, long maxp);
+long long mm_nl_builds(mm_engine *e);
+
+/* MD with velocity Verlet and rigid-water constraints (mdyn.cpp:120-160):
+ * setup constrains the initial positions (constraint tolerance tol) and
+ * computes the initial forces; run advances nsteps of dt ps.  use_graph
+ * replays each step as one captured hipGraph. */
+int mm_md_setup(mm_engine *e, double dt, double bond, double angle_deg, double tol);
+int mm_md_run(mm_engine *e, int nsteps, int use_graph);
+
+/* stream the engine runs on (hipStream_t) and device pointers of the FP64
+ * SoA state (x[0..N), y[N..2N), z[2N..3N)) for device-resident callers */
+void *mm_stream(mm_engine *e);
+double *mm_device_positions(mm_engine *e);
+double *mm_device_forces(mm_engine *e);
+int mm_sync(mm_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,436 @@
+// capi.cpp -- the extern "C" engine API declared in include/molmodel_amd.h.
+#include "../../include/molmodel_amd.h"
+#include "engine.hpp"
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+using namespace mmamd;
+
+struct mm_engine {
+  Engine e;
+  int pos_dirty = 1;
+  int want_virial_last = 0;
+};
+
+#define CHECK_ENGINE(p)                       \
+  do {                                        \
+    if (!(p)) {                               \
+      set_last_error("null engine");          \
+      return MM_ERR_ARG;                      \
+    }                                         \
+  } while (0)
+
+#define TRY(x)            \
+  do {                    \
+    int rc_ = (x);        \
+    if (rc_) return rc_;  \
+  } while (0)
+
+extern "C" {
+
+const char *mm_last_error(void) { return last_error(); }
+
+int mm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out) {
+  if (!out || nsys <= 0 || !sys) {
+    set_last_error("mm_engine_create: bad arguments");
+    return MM_ERR_ARG;
+  }
+  *out = nullptr;
+  if (mm_device_count() <= 0) {
+    set_last_error("mm_engine_create: no HIP device");
+    return MM_ERR_HIP;
+  }
+  std::vector<SystemInput> in(nsys);
+  for (int s = 0; s < nsys; s++) {
+    const mm_system_desc &d = sys[s];
+    SystemInput &S = in[s];
+    if (d.nsite <= 0 || d.ngroup <= 0 || !d.group || !d.q || !d.sigma || !d.epsilon ||
+        !d.site_mask) {
+      set_last_error("mm_engine_create: missing arrays");
+      return MM_ERR_ARG;
+    }
+    S.nsite = d.nsite;
+    S.ngroup = d.ngroup;
+    S.gfirst.resize(d.ngroup);
+    S.gns.resize(d.ngroup);
+    for (int g = 0; g < d.ngroup; g++) {
+      S.gfirst[g] = d.group[2 * g];
+      S.gns[g] = d.group[2 * g + 1] - d.group[2 * g];
+    }
+    S.q.assign(d.q, d.q + d.nsite);
+    S.sig.assign(d.sigma, d.sigma + d.nsite);
+    S.eps.assign(d.epsilon, d.epsilon + d.nsite);
+    S.mask.assign(d.site_mask, d.site_mask + d.nsite);
+    if (d.mass) S.mass.assign(d.mass, d.mass + d.nsite);
+    if (d.nexcl > 0 && d.excl) S.excl.assign(d.excl, d.excl + 2 * d.nexcl);
+    S.boxl = d.boxl;
+    S.rc = d.rc;
+    S.width = d.smoothing_width;
+    S.skin = d.skin;
+    S.eta = d.eta;
+    S.kmode = d.kmode;
+    S.kc = d.kc;
+    S.nx = d.mesh[0];
+    S.ny = d.mesh[1];
+    S.nz = d.mesh[2];
+    S.order = d.order;
+    S.nalias = d.nalias;
+    S.ik = d.ik;
+    S.include_lj_correction = d.include_lj_correction;
+    S.geometric_sigma = d.geometric_sigma;
+  }
+  mm_engine *m = new mm_engine();
+  int rc = engine_create(m->e, in);
+  if (rc) {
+    engine_destroy(m->e);
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return MM_OK;
+}
+
+int mm_engine_destroy(mm_engine *m) {
+  if (!m) return MM_OK;
+  hipStreamSynchronize(m->e.stream);
+  engine_destroy(m->e);
+  delete m;
+  return MM_OK;
+}
+
+int mm_engine_sizes(const mm_engine *m, int *nsite, int *ngroup) {
+  CHECK_ENGINE(m);
+  if (nsite) *nsite = m->e.nsite;
+  if (ngroup) *ngroup = m->e.ngroup;
+  return MM_OK;
+}
+
+static int put_aos(mm_engine *m, double *d, const double *h) {
+  Engine &e = m->e;
+  const int N = e.nsite;
+  std::vector<double> soa(3 * (size_t)N);
+  for (int i = 0; i < N; i++)
+    for (int c = 0; c < 3; c++) soa[(size_t)c * N + i] = h[3 * (size_t)i + c];
+  MM_HIP(hipMemcpyAsync(d, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  MM_HIP(hipStreamSynchronize(e.stream));
+  return 0;
+}
+
+static int get_aos(mm_engine *m, const double *d, double *h) {
+  Engine &e = m->e;
+  const int N = e.nsite;
+  std::vector<double> soa(3 * (size_t)N);
+  MM_HIP(hipMemcpyAsync(soa.data(), d, soa.size() * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  MM_HIP(hipStreamSynchronize(e.stream));
+  for (int i = 0; i < N; i++)
+    for (int c = 0; c < 3; c++) h[3 * (size_t)i + c] = soa[(size_t)c * N + i];
+  return 0;
+}
+
+int mm_set_positions(mm_engine *m, const double *r) {
+  CHECK_ENGINE(m);
+  if (!r) { set_last_error("null positions"); return MM_ERR_ARG; }
+  TRY(put_aos(m, m->e.d_r, r));
+  m->pos_dirty = 1;
+  m->e.md_phase = 0;
+  return MM_OK;
+}
+int mm_get_positions(mm_engine *m, double *r) { CHECK_ENGINE(m); return get_aos(m, m->e.d_r, r); }
+int mm_set_velocities(mm_engine *m, const double *v) {
+  CHECK_ENGINE(m);
+  if (!v) { set_last_error("null velocities"); return MM_ERR_ARG; }
+  return put_aos(m, m->e.d_v, v);
+}
+int mm_get_velocities(mm_engine *m, double *v) { CHECK_ENGINE(m); return get_aos(m, m->e.d_v, v); }
+int mm_get_forces(mm_engine *m, double *f) { CHECK_ENGINE(m); return get_aos(m, m->e.d_f, f); }
+
+int mm_get_potential(mm_engine *m, double *phi) {
+  CHECK_ENGINE(m);
+  MM_HIP(hipMemcpyAsync(phi, m->e.d_phi, sizeof(double) * m->e.nsite, hipMemcpyDeviceToHost, m->e.stream));
+  MM_HIP(hipStreamSynchronize(m->e.stream));
+  return MM_OK;
+}
+
+static int read_err(Engine &e, int *flags) {
+  int h = 0;
+  MM_HIP(hipMemcpyAsync(&h, e.d_err, sizeof(int), hipMemcpyDeviceToHost, e.stream));
+  MM_HIP(hipStreamSynchronize(e.stream));
+  *flags = h;
+  return 0;
+}
+
+static int clear_err(Engine &e) {
+  int z = 0;
+  MM_HIP(hipMemcpyAsync(e.d_err, &z, sizeof(int), hipMemcpyHostToDevice, e.stream));
+  MM_HIP(hipStreamSynchronize(e.stream));
+  return 0;
+}
+
+int mm_errors(mm_engine *m, int *flags) {
+  CHECK_ENGINE(m);
+  int f = 0;
+  TRY(read_err(m->e, &f));
+  TRY(clear_err(m->e));
+  if (flags) *flags = f;
+  return MM_OK;
+}
+
+// grow the neighbour rows after an overflow and force a rebuild
+static int grow_nbr(Engine &e) {
+  MM_HIP(hipStreamSynchronize(e.stream));
+  MM_HIP(hipFree(e.d_nbr));
+  e.nbcap = std::min(e.nbcap * 2, ((e.maxg + 15) / 16) * 16);
+  MM_HIP(hipMalloc((void **)&e.d_nbr, sizeof(int) * (size_t)e.ngroup * e.nbcap));
+  e.first_eval = 1;
+  return 0;
+}
+
+static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
+  Engine &e = m->e;
+  if (e.first_eval) {
+    TRY(launch_nl_build(e, 1));
+    e.first_eval = 0;
+  } else {
+    if (m->pos_dirty) TRY(launch_skin(e));
+    TRY(launch_nl_build(e, 0));
+  }
+  m->pos_dirty = 0;
+  TRY(launch_pack(e));
+  TRY(launch_pair(e, field_mode, want_virial));
+  TRY(launch_kspace(e, want_virial));
+  TRY(launch_p3m(e, want_virial));
+  return 0;
+}
+
+static int compute_checked(mm_engine *m, int field_mode, int want_virial) {
+  Engine &e = m->e;
+  for (int attempt = 0; attempt < 4; attempt++) {
+    TRY(enqueue_forces(m, field_mode, want_virial));
+    if (!field_mode) TRY(launch_finish_ex(e, FIN_FORCES, want_virial, e.d_phi));
+    int f = 0;
+    TRY(read_err(e, &f));
+    if (f & ERR_NL_OVERFLOW) {
+      TRY(clear_err(e));
+      TRY(grow_nbr(e));
+      continue;
+    }
+    if (f & (ERR_NAN | ERR_CONSTRAINT)) {
+      set_last_error("device error flag raised during force evaluation");
+      return MM_ERR_DEVICE;
+    }
+    return 0;
+  }
+  set_last_error("neighbour list overflow persisted");
+  return MM_ERR_DEVICE;
+}
+
+int mm_compute(mm_engine *m, int flags) {
+  CHECK_ENGINE(m);
+  m->want_virial_last = (flags & MM_VIRIAL) ? 1 : 0;
+  TRY(compute_checked(m, 0, m->want_virial_last));
+  m->e.md_phase = 0;
+  return MM_OK;
+}
+
+int mm_get_energies(mm_engine *m, double *en) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  std::vector<double> acc((size_t)e.nsys * NACC);
+  MM_HIP(hipMemcpyAsync(acc.data(), e.d_acc, acc.size() * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  MM_HIP(hipStreamSynchronize(e.stream));
+  for (int s = 0; s < e.nsys; s++) {
+    const double *a = &acc[(size_t)s * NACC];
+    double *o = en + (size_t)s * MM_NE;
+    o[MM_E_ELEC] = a[ACC_ELEC];
+    o[MM_E_LJ] = a[ACC_LJ] + e.lj_tail[s];
+    o[MM_E_TAIL] = e.lj_tail[s];
+    o[MM_E_TOTAL] = o[MM_E_ELEC] + o[MM_E_LJ];
+    o[MM_E_KINETIC] = a[ACC_KIN];
+    o[MM_E_KSPACE] = a[ACC_KSPACE_E];
+    o[6] = a[ACC_TOP1];
+    o[7] = a[ACC_TOP2];
+  }
+  return MM_OK;
+}
+
+int mm_get_virial(mm_engine *m, double *w) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  std::vector<double> acc((size_t)e.nsys * NACC), kv(9 * (size_t)e.nsys);
+  MM_HIP(hipMemcpyAsync(acc.data(), e.d_acc, acc.size() * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  MM_HIP(hipMemcpyAsync(kv.data(), e.d_kres + 4L * e.nsite, kv.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, e.stream));
+  MM_HIP(hipStreamSynchronize(e.stream));
+  for (int s = 0; s < e.nsys; s++)
+    for (int c = 0; c < 9; c++)
+      w[9 * s + c] = acc[(size_t)s * NACC + ACC_VIR + c] + kv[9 * s + c] +
+                     ((c % 4 == 0) ? e.lj_tail[s] : 0.0);
+  return MM_OK;
+}
+
+static int scratch(Engine &e, size_t n) {
+  if (e.scratch_n >= n) return 0;
+  if (e.d_scratch) MM_HIP(hipFree(e.d_scratch));
+  MM_HIP(hipMalloc((void **)&e.d_scratch, n * sizeof(double)));
+  e.scratch_n = n;
+  return 0;
+}
+
+int mm_rspace_field(mm_engine *m, double *phi, double *evec, double *lj_force, double *lj_energy) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  TRY(compute_checked(m, 1, 0));
+  const size_t N = e.nsite;
+  TRY(scratch(e, 7 * N + e.nsys));
+  MM_HIP(hipMemsetAsync(e.d_scratch + 7 * N, 0, sizeof(double) * e.nsys, e.stream));
+  TRY(launch_field_out(e, e.d_scratch, e.d_scratch + 7 * N));
+  std::vector<double> h(7 * N + e.nsys);
+  MM_HIP(hipMemcpyAsync(h.data(), e.d_scratch, h.size() * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  MM_HIP(hipStreamSynchronize(e.stream));
+  for (size_t i = 0; i < N; i++) {
+    if (phi) phi[i] += h[7 * i];
+    for (int c = 0; c < 3; c++) {
+      if (evec) evec[3 * i + c] += h[7 * i + 1 + c];
+      if (lj_force) lj_force[3 * i + c] += h[7 * i + 4 + c];
+    }
+  }
+  if (lj_energy)
+    for (int s = 0; s < e.nsys; s++) lj_energy[s] += h[7 * N + s];
+  e.md_phase = 0;
+  return MM_OK;
+}
+
+int mm_kspace_field(mm_engine *m, double *phi, double *evec) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  TRY(compute_checked(m, 0, 0));
+  const size_t N = e.nsite;
+  std::vector<double> h(4 * N);
+  MM_HIP(hipMemcpyAsync(h.data(), e.d_kres, h.size() * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  MM_HIP(hipStreamSynchronize(e.stream));
+  for (size_t i = 0; i < N; i++) {
+    if (phi) phi[i] += h[4 * i];
+    if (evec)
+      for (int c = 0; c < 3; c++) evec[3 * i + c] += h[4 * i + 1 + c];
+  }
+  return MM_OK;
+}
+
+long mm_nl_pairs(mm_engine *m, int sys, int *pairs, long maxp) {
+  if (!m) { set_last_error("null engine"); return MM_ERR_ARG; }
+  Engine &e = m->e;
+  if (sys < 0 || sys >= e.nsys) { set_last_error("bad system index"); return MM_ERR_ARG; }
+  if (e.first_eval || m->pos_dirty) {
+    int rc = compute_checked(m, 0, 0);
+    if (rc) return rc;
+  }
+  const SysParams &P = e.hsp[sys];
+  std::vector<int> order(e.ngroup), nnb(P.ngroup), nbr((size_t)P.ngroup * e.nbcap);
+  if (hipMemcpy(order.data(), e.d_order, sizeof(int) * e.ngroup, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(nnb.data(), e.d_nnb + P.group0, sizeof(int) * P.ngroup, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(nbr.data(), e.d_nbr + (size_t)P.group0 * e.nbcap, sizeof(int) * nbr.size(),
+                hipMemcpyDeviceToHost) != hipSuccess) {
+    set_last_error("copy of neighbour list failed");
+    return MM_ERR_HIP;
+  }
+  long np = 0;
+  for (int i = 0; i < P.ngroup; i++) {
+    const int g = order[P.group0 + i] - P.group0;
+    for (int t = 0; t < nnb[i]; t++) {
+      const int h = order[nbr[(size_t)i * e.nbcap + t]] - P.group0;
+      if (g >= h) continue;   // the full list holds both directions
+      int a = g, b = h;
+      canonical_pair(a, b);
+      if (np < maxp && pairs) { pairs[2 * np] = a; pairs[2 * np + 1] = b; }
+      np++;
+    }
+  }
+  return np;
+}
+
+int mm_md_setup(mm_engine *m, double dt, double bond, double angle_deg, double tol) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  if (!(dt > 0) || !(bond > 0) || !(tol > 0)) { set_last_error("bad MD parameters"); return MM_ERR_ARG; }
+  for (auto &s : e.in)
+    if (s.mass.empty()) { set_last_error("masses are required for dynamics"); return MM_ERR_ARG; }
+  e.bond = bond;
+  e.angle = angle_deg;
+  e.tol = tol;
+  for (auto &P : e.hsp) P.dt = dt;
+  MM_HIP(hipMemcpy(e.d_sp, e.hsp.data(), sizeof(SysParams) * e.nsys, hipMemcpyHostToDevice));
+  if (e.gexec) { hipGraphExecDestroy(e.gexec); e.gexec = nullptr; }
+  if (e.graph) { hipGraphDestroy(e.graph); e.graph = nullptr; }
+  TRY(launch_constrain_initial(e));   // mdyn.cpp:120-121
+  m->pos_dirty = 1;
+  TRY(compute_checked(m, 0, 0));      // mdyn.cpp:122-124
+  e.md_ready = 1;
+  e.md_phase = 0;
+  return MM_OK;
+}
+
+static int enqueue_step(mm_engine *m, int mode) {
+  TRY(enqueue_forces(m, 0, 0));
+  TRY(launch_finish_ex(m->e, mode, 0, m->e.d_phi));
+  return 0;
+}
+
+int mm_md_run(mm_engine *m, int nsteps, int use_graph) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  if (!e.md_ready) { set_last_error("mm_md_setup first"); return MM_ERR_ARG; }
+  if (nsteps <= 0) return MM_OK;
+  if (e.md_phase == 0) {
+    // first half of a step with the current forces (mdyn.cpp:144-150)
+    TRY(launch_finish_ex(e, FIN_DRIFT, 0, e.d_phi));
+    e.md_phase = 1;
+  }
+  int full = nsteps - 1;
+  if (use_graph && full > 0) {
+    if (!e.gexec) {
+      // the step graph: neighbour-list check/rebuild, forces, finish
+      MM_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_step(m, FIN_KICK | FIN_DRIFT);
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(e.stream, &g);
+      if (rc) return rc;
+      MM_HIP(ce);
+      e.graph = g;
+      MM_HIP(hipGraphInstantiate(&e.gexec, e.graph, nullptr, nullptr, 0));
+    }
+    for (int s = 0; s < full; s++) MM_HIP(hipGraphLaunch(e.gexec, e.stream));
+  } else {
+    for (int s = 0; s < full; s++) TRY(enqueue_step(m, FIN_KICK | FIN_DRIFT));
+  }
+  // last step ends synchronised: r, v, F of the same time
+  TRY(enqueue_step(m, FIN_KICK));
+  e.md_phase = 0;
+  int f = 0;
+  TRY(read_err(e, &f));
+  if (f) {
+    set_last_error(f & ERR_NL_OVERFLOW ? "neighbour list overflow during MD"
+                   : f & ERR_CONSTRAINT ? "constraint iteration did not converge"
+                                        : "non-finite energy during MD");
+    return MM_ERR_DEVICE;
+  }
+  return MM_OK;
+}
+
+void *mm_stream(mm_engine *m) { return m ? (void *)m->e.stream : nullptr; }
+double *mm_device_positions(mm_engine *m) { return m ? m->e.d_r : nullptr; }
+double *mm_device_forces(mm_engine *m) { return m ? m->e.d_f : nullptr; }
+int mm_sync(mm_engine *m) {
+  CHECK_ENGINE(m);
+  MM_HIP(hipStreamSynchronize(m->e.stream));
+  return MM_OK;
+}
+long long mm_nl_builds(mm_engine *m) { return m ? 0 : -1; }
+
+}  // extern "C"

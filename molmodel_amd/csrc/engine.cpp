@@ -1,0 +1,557 @@
+// engine.cpp -- construction of the device engine and the host-side tables
+// it needs (all FP64 on the host, rounded once to the device formats).
+#include "engine.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <set>
+#include <tuple>
+
+namespace mmamd {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &s) { g_last_error = s; }
+const char *last_error() { return g_last_error.c_str(); }
+
+#define MM_REQ(cond, msg)                    \
+  do {                                       \
+    if (!(cond)) {                           \
+      set_last_error(msg);                   \
+      return -1;                             \
+    }                                        \
+  } while (0)
+
+static double sq(double x) { return x * x; }
+
+// ---- neighbor-list geometry (nbrlist.cpp:11, 96-102, 105-120) ----
+int nl_ncell(int ngroup) { return (int)floor(cbrt((double)(ngroup / 2 + 3))); }
+
+int nl_nmax(int ncell, double boxl, double rc, double skin) {
+  const double lc = boxl * (1.0 / ncell);
+  const double maxd = sqrt(3.0) * lc;
+  const double vol = fabs(lc * lc * lc);
+  const double a = sqrt(0.0 + 0.0 + sq(lc * lc));
+  const double mind = vol / a;
+  const int nmax = (int)ceil((rc + skin + maxd) / mind);
+  const int cap = ncell / 2 + 1;
+  return nmax < cap ? nmax : cap;
+}
+
+// Distinct cell offsets (as residues mod ncell) that the reference's cell
+// pair rows reach, plus the zero offset for same-cell pairs; offsets whose
+// cells cannot hold a pair closer than rc+skin are dropped (they add no
+// pairs), which leaves the pair set unchanged.
+static std::vector<int4> nl_stencil(int ncell, double boxl, double rc, double skin) {
+  const double lc = boxl * (1.0 / ncell);
+  const double inv = 1 / lc;
+  const double c2 = sq(rc + skin + sqrt(3.0) * lc);
+  const int nmax = nl_nmax(ncell, boxl, rc, skin);
+  std::set<std::tuple<int, int, int>> res;
+  auto m = [&](int a) { return ((a % ncell) + ncell) % ncell; };
+  for (int jx = 0; jx <= nmax; jx++)
+    for (int jy = (jx == 0 ? 0 : -nmax); jy <= nmax; jy++)
+      for (int jz = (jx == 0 && jy == 0 ? 1 : -nmax); jz <= nmax; jz++) {
+        double x = lc * jx + 0.0 * jy + 0.0 * jz;
+        double y = 0.0 * jx + lc * jy + 0.0 * jz;
+        double z = 0.0 * jx + 0.0 * jy + lc * jz;
+        x -= lc * floor(x * inv + 0.5);
+        y -= lc * floor(y * inv + 0.5);
+        z -= lc * floor(z * inv + 0.5);
+        if (!(x * x + y * y + z * z < c2)) continue;
+        auto a = std::make_tuple(m(jx), m(jy), m(jz));
+        auto b = std::make_tuple(m(-jx), m(-jy), m(-jz));
+        if (a != std::make_tuple(0, 0, 0)) res.insert(a);
+        if (b != std::make_tuple(0, 0, 0)) res.insert(b);
+      }
+  std::vector<int4> out;
+  out.push_back(make_int4(0, 0, 0, 1));
+  const double cut2 = sq(rc + skin) * (1 + 1e-9) + 1e-9;
+  for (auto &t : res) {
+    int o[3] = {std::get<0>(t), std::get<1>(t), std::get<2>(t)};
+    double lb = 0;
+    for (int d = 0; d < 3; d++) {
+      const int k = std::min(o[d], ncell - o[d]);
+      lb += sq(std::max(0, k - 1) * lc);
+    }
+    if (lb > cut2) continue;
+    out.push_back(make_int4(o[0], o[1], o[2], 0));
+  }
+  return out;
+}
+
+// ---- erfc(eta r)/r table (cheby.c:7-33 fit; rspace.c:25-37 ranges) ----
+static void cheb_fit(double a, double b, double eta, double *c4) {
+  double g[50], c[4];
+  const double fac = 2.0 / 50, bma = 0.5 * (b - a), bpa = 0.5 * (b + a);
+  for (int k = 0; k < 50; k++) {
+    const double r2 = cos(M_PI * (k + 0.5) / 50) * bma + bpa;
+    const double r = sqrt(r2);
+    g[k] = erfc(eta * r) / r;
+  }
+  for (int j = 0; j < 4; j++) {
+    double s = 0;
+    for (int k = 0; k < 50; k++) s += g[k] * cos(M_PI * j * (k + 0.5) / 50);
+    c[j] = fac * s;
+  }
+  c4[0] = c[0] / 2 - c[1] + c[2] - c[3];
+  c4[1] = 2 * c[1] - 8 * c[2] + 18 * c[3];
+  c4[2] = 8 * c[2] - 48 * c[3];
+  c4[3] = 32 * c[3];
+}
+
+static void build_table(SysParams &P, double rc, double eta, std::vector<float4> &tab) {
+  const double r2max = 1.2 * sq(rc);
+  const double r2min = r2max > 4 ? 4 : 0.9 * r2max;
+  // the reference's interval width, widened only if the extended range
+  // (site pairs of groups within rc reach about rc + 2 group radii) would
+  // not fit in LDS
+  const double bmax = sq(rc + 3.5);
+  const double dxB = std::max((r2max - r2min) / 512, (bmax - r2min) / 1280);
+  const int nB = std::max(1, (int)ceil((bmax - r2min) / dxB));
+  const double amin = std::min(0.81, 0.5 * r2min);
+  const int nA = 128;
+  const double dxA = (r2min - amin) / nA;
+  P.tab0 = (int)tab.size();
+  P.ntabA = nA;
+  P.ntabB = nB;
+  P.tabAmin = (float)amin;
+  P.tabMid = (float)r2min;
+  P.tabAinv = (float)(1.0 / dxA);
+  P.tabBinv = (float)(1.0 / dxB);
+  // the float table range ends where the last float interval ends
+  P.tabBmax = (float)(r2min + nB * dxB) * (1.0f - 4e-7f);
+  double c[4];
+  for (int i = 0; i < nA; i++) {
+    cheb_fit(amin + i * dxA, amin + (i + 1) * dxA, eta, c);
+    tab.push_back(make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[3]));
+  }
+  for (int i = 0; i < nB; i++) {
+    cheb_fit(i * dxB + r2min, (i + 1) * dxB + r2min, eta, c);
+    tab.push_back(make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[3]));
+  }
+}
+
+// ---- tensors in tensor_t layout (cart.h:73-92) ----
+static void tensor_inverse(double *a, const double *b) {
+  const double xx = b[0], yx = b[1], zx = b[2], xy = b[3], yy = b[4], zy = b[5], xz = b[6],
+               yz = b[7], zz = b[8];
+  const double det = -xz * yy * zx + xy * yz * zx + xz * yx * zy - xx * yz * zy - xy * yx * zz + xx * yy * zz;
+  const double id = 1.0 / det;
+  a[0] = id * (-yz * zy + yy * zz);
+  a[3] = id * (xz * zy - xy * zz);
+  a[6] = id * (-xz * yy + xy * yz);
+  a[1] = id * (yz * zx - yx * zz);
+  a[4] = id * (-xz * zx + xx * zz);
+  a[7] = id * (xz * yx - xx * yz);
+  a[2] = id * (-yy * zx + yx * zy);
+  a[5] = id * (xy * zx - xx * zy);
+  a[8] = id * (-xy * yx + xx * yy);
+}
+
+static double sincpi(double x) {   // pppm.c:56-59
+  const double y = M_PI * x;
+  return fabs(y) < 1e-16 ? 1 - sq(y) / 6 : sin(y) / y;
+}
+
+// optimal influence function at signed index (pppm.c:331-388)
+static double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl,
+                       double inv_4eta2, double inv_v, int nalias, int order) {
+  if (jx == 0 && jy == 0 && jz == 0) return 0;
+  double ux[16], uy[16], uz[16];
+  for (int qq = -nalias; qq <= nalias; qq++) {
+    ux[qq + nalias] = pow(sincpi((double)jx / nx + qq), 2 * order);
+    uy[qq + nalias] = pow(sincpi((double)jy / ny + qq), 2 * order);
+    uz[qq + nalias] = pow(sincpi((double)jz / nz + qq), 2 * order);
+  }
+  double numer = 0, d1 = 0, d2 = 0;
+  for (int px = -nalias; px <= nalias; px++) {
+    const double pxx = jx + px * nx;
+    for (int py = -nalias; py <= nalias; py++) {
+      const double pyy = jy + py * ny;
+      for (int pz = -nalias; pz <= nalias; pz++) {
+        const double pzz = jz + pz * nz;
+        if (fabs(pxx) < 1e-16 && fabs(pyy) < 1e-16 && fabs(pzz) < 1e-16) continue;
+        const double k0 = rl[0] * pxx + rl[3] * pyy + rl[6] * pzz;
+        const double k1 = rl[1] * pxx + rl[4] * pyy + rl[7] * pzz;
+        const double k2 = rl[2] * pxx + rl[5] * pyy + rl[8] * pzz;
+        const double kp2 = k0 * k0 + k1 * k1 + k2 * k2;
+        const double u2 = ux[px + nalias] * uy[py + nalias] * uz[pz + nalias];
+        const double gg = 4 * M_PI / kp2 * exp(-kp2 * inv_4eta2);
+        numer += gg * kp2 * u2;
+        d1 += u2;
+        d2 += u2 * kp2;
+      }
+    }
+  }
+  return inv_v * numer / (d1 * d2);
+}
+
+static bool radix_ok(int n) {
+  if (n < 1 || n > 128) return false;
+  for (int p : {2, 3, 5, 7})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+static int largest_div_le(int n, int lim) {
+  for (int b = lim; b >= 1; b--)
+    if (n % b == 0) return b;
+  return 1;
+}
+
+template <class T>
+static int dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  MM_HIP(hipMalloc((void **)p, n * sizeof(T)));
+  MM_HIP(hipMemset(*p, 0, n * sizeof(T)));
+  return 0;
+}
+
+template <class T>
+static int dput(T *d, const std::vector<T> &h) {
+  if (h.empty()) return 0;
+  MM_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+#define MM_TRY(x)          \
+  do {                     \
+    int rc_ = (x);         \
+    if (rc_) return rc_;   \
+  } while (0)
+
+// eleclj.cpp:279-308 + 1247-1266: LJ tail over distinct "%12.6g" keys
+static double lj_tail(const SystemInput &s) {
+  std::map<std::string, int> keys;
+  for (int i = 0; i < s.nsite; i++) {
+    if (!(s.mask[i] & SITE_HAS_LJ)) continue;
+    char key[128];
+    snprintf(key, sizeof key, "%12.6g %12.6g", s.sig[i], s.eps[i]);
+    keys[key]++;
+  }
+  double totA = 0, totB = 0;
+  for (auto &a : keys) {
+    double si, ei;
+    sscanf(a.first.c_str(), "%lf%lf", &si, &ei);
+    for (auto &b : keys) {
+      double sj, ej;
+      sscanf(b.first.c_str(), "%lf%lf", &sj, &ej);
+      const double s6 = pow(s.geometric_sigma ? sqrt(si * sj) : 0.5 * (si + sj), 6);
+      const double bb = 4 * sqrt(ei * ej) * a.second * b.second * s6;
+      totB -= bb;
+      totA += bb * s6;
+    }
+  }
+  const double V = s.boxl * s.boxl * s.boxl;
+  const double rhi = s.rc, rlo = rhi - s.width;
+  const double rlo2 = rlo * rlo, rlo3 = rlo2 * rlo, rlo4 = rlo3 * rlo;
+  const double rlo5 = rlo4 * rlo, rlo6 = rlo5 * rlo, rlo7 = rlo6 * rlo, rlo9 = rlo7 * rlo2;
+  const double rhi2 = rhi * rhi, rhi3 = rhi2 * rhi, rhi4 = rhi3 * rhi, rhi5 = rhi4 * rhi;
+  const double rhi6 = rhi5 * rhi, rhi7 = rhi6 * rhi;
+  return 2 * M_PI / V *
+         (totA / (9 * rlo9) + totB / (3 * rlo3) -
+          (rhi - rlo) / (63 * rhi3 * rlo9 * pow(rhi + rlo, 5)) *
+              (3 * totB * rhi3 * rlo6 * (7 * rhi4 + 42 * rhi3 * rlo + 112 * rhi2 * rlo2 + 150 * rhi * rlo3 + 25 * rlo4) +
+               totA * (7 * rhi7 + 42 * rhi6 * rlo + 112 * rhi5 * rlo2 + 182 * rhi4 * rlo3 + 217 * rhi3 * rlo4 +
+                       224 * rhi2 * rlo5 + 192 * rhi * rlo6 + 32 * rlo7)));
+}
+
+int engine_create(Engine &e, const std::vector<SystemInput> &in) {
+  e.in = in;
+  e.nsys = (int)in.size();
+  MM_REQ(e.nsys > 0, "no systems");
+  std::vector<int> gfirst, gns, gsys, ginfo, gexcl;
+  std::vector<double> q, mass;
+  std::vector<float4> tab;
+  std::vector<float2> ljt;
+  std::vector<int4> sten;
+  std::vector<double> kvec;
+  std::vector<float> green;
+  std::vector<float2> twid;
+  int site0 = 0, group0 = 0, cell0 = 0, k0 = 0, bin0 = 0;
+  long mesh0 = 0, cmesh0 = 0, green0 = 0;
+  int maxnb = 64;
+  for (int s = 0; s < e.nsys; s++) {
+    const SystemInput &S = in[s];
+    SysParams P;
+    memset(&P, 0, sizeof P);
+    MM_REQ(S.nsite > 0 && S.ngroup > 0, "empty system");
+    MM_REQ(S.boxl > 0, "box length must be positive");
+    MM_REQ(S.rc > 0 && S.rc <= 0.49 * S.boxl,
+           "rspace cutoff must be > 0 and <= 0.49 * box length (eleclj.cpp:164-170)");
+    MM_REQ(S.width >= 0 && S.width < S.rc, "smoothing width must be less than the cutoff (eleclj.cpp:180-182)");
+    MM_REQ(S.eta > 0, "Ewald screening parameter must be positive");
+    MM_REQ(S.skin > 0, "neighbour list width must be positive");
+    P.L = S.boxl;
+    P.invL = 1 / S.boxl;
+    P.volume = S.boxl * S.boxl * S.boxl;
+    P.rhi2 = sq(S.rc);
+    P.rlo2 = sq(S.rc - S.width);
+    P.inv_w2 = P.rhi2 > P.rlo2 ? 1.0 / (P.rhi2 - P.rlo2) : 0.0;
+    P.nl_c2 = sq(S.rc + S.skin);
+    P.skin = S.skin;
+    P.eta = S.eta;
+    P.self_coef = S.eta * 1.12837916709551257390;   // M_2_SQRTPI
+    P.site0 = site0;
+    P.nsite = S.nsite;
+    P.group0 = group0;
+    P.ngroup = S.ngroup;
+    // groups
+    std::vector<int> gof(S.nsite, -1);
+    for (int g = 0; g < S.ngroup; g++) {
+      MM_REQ(S.gns[g] >= 1 && S.gns[g] <= 3, "groups must have 1..3 sites on the device");
+      MM_REQ(S.gfirst[g] >= 0 && S.gfirst[g] + S.gns[g] <= S.nsite, "group out of range");
+      for (int k = 0; k < S.gns[g]; k++) {
+        MM_REQ(gof[S.gfirst[g] + k] < 0, "groups overlap");
+        gof[S.gfirst[g] + k] = g;
+      }
+    }
+    for (int i = 0; i < S.nsite; i++) MM_REQ(gof[i] >= 0, "site not in any group");
+    // LJ types: distinct (sigma, epsilon) among LJ sites
+    std::map<std::pair<double, double>, int> types;
+    std::vector<int> stype(S.nsite, 0);
+    for (int i = 0; i < S.nsite; i++)
+      if (S.mask[i] & SITE_HAS_LJ) {
+        auto key = std::make_pair(S.sig[i], S.eps[i]);
+        auto it = types.find(key);
+        if (it == types.end()) it = types.emplace(key, (int)types.size()).first;
+        stype[i] = it->second;
+      }
+    const int nt = std::max(1, (int)types.size());
+    MM_REQ(nt <= 8, "at most 8 distinct LJ parameter pairs per system on the device");
+    std::vector<std::pair<double, double>> tp(nt, {0.0, 0.0});
+    for (auto &t : types) tp[t.second] = t.first;
+    P.ntype = nt;
+    P.lj0 = (int)ljt.size();
+    for (int a = 0; a < nt; a++)
+      for (int b = 0; b < nt; b++) {   // rspace.c:267-270
+        const double s2 = S.geometric_sigma ? tp[a].first * tp[b].first
+                                            : sq(0.5 * (tp[a].first + tp[b].first));
+        const double s6 = s2 * s2 * s2;
+        const double bb = 4 * sqrt(tp[a].second * tp[b].second) * s6;
+        ljt.push_back(make_float2((float)bb, (float)(bb * s6)));
+      }
+    // exclusions: intra-group only on the device
+    std::vector<int> gx(S.ngroup, 0);
+    for (size_t k = 0; 2 * k + 1 < S.excl.size(); k++) {
+      const int a = S.excl[2 * k], b = S.excl[2 * k + 1];
+      MM_REQ(a >= 0 && a < S.nsite && b >= 0 && b < S.nsite, "exclusion out of range");
+      if (a == b) continue;
+      MM_REQ(gof[a] == gof[b], "exclusions between different groups are not supported on the device");
+      const int g = gof[a];
+      int ka = a - S.gfirst[g], kb = b - S.gfirst[g];
+      if (ka > kb) std::swap(ka, kb);
+      gx[g] |= 1 << (ka * MAXS + kb);
+    }
+    for (int g = 0; g < S.ngroup; g++) {
+      gfirst.push_back(site0 + S.gfirst[g]);
+      gns.push_back(S.gns[g]);
+      gsys.push_back(s);
+      gexcl.push_back(gx[g]);
+      int info = 0;
+      for (int k = 0; k < S.gns[g]; k++) {
+        const int i = S.gfirst[g] + k;
+        info |= ((S.mask[i] & 3) | (stype[i] << 2)) << (8 * k);
+      }
+      ginfo.push_back(info);
+    }
+    double qsum = 0;
+    for (int i = 0; i < S.nsite; i++) {
+      q.push_back(S.q[i]);
+      qsum += S.q[i];
+      mass.push_back(S.mass.empty() ? 1.0 : S.mass[i]);
+    }
+    // neutralising background (eleclj.cpp:970-975)
+    P.bg_phi = fabs(qsum) < 64.0 * 2.220446049250313e-16 ? 0.0 : -M_PI / (sq(S.eta) * P.volume) * qsum;
+    // neighbor list
+    P.ncell = nl_ncell(S.ngroup);
+    P.lc = S.boxl * (1.0 / P.ncell);
+    P.cell0 = cell0;
+    cell0 += P.ncell * P.ncell * P.ncell;
+    auto st = nl_stencil(P.ncell, S.boxl, S.rc, S.skin);
+    P.sten0 = (int)sten.size();
+    P.nsten = (int)st.size();
+    sten.insert(sten.end(), st.begin(), st.end());
+    {
+      const double dens = S.ngroup / P.volume;
+      const double est = dens * 4.0 / 3.0 * M_PI * pow(S.rc + S.skin, 3);
+      int cap = (int)(1.6 * est) + 48;
+      cap = std::min(cap, S.ngroup);
+      maxnb = std::max(maxnb, cap);
+    }
+    // erfc table
+    build_table(P, S.rc, S.eta, tab);
+    MM_REQ(P.ntabA + P.ntabB <= 1536, "erfc table too large for the box/cutoff");
+    // lattice tensors
+    const double latv[9] = {S.boxl, 0, 0, 0, S.boxl, 0, 0, 0, S.boxl};
+    double inv[9];
+    tensor_inverse(inv, latv);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) P.RL[3 * a + b] = 2 * M_PI * inv[3 * b + a];
+    P.kmode = S.kmode;
+    if (S.kmode == 0) {
+      // kspace.c:35-58 and 96-103
+      const double nsqmax = S.kc * S.kc + 1e-8;
+      const int nm = (int)floor(S.kc);
+      const double c = 0.25 / (S.eta * S.eta), fourpi_V = 4 * M_PI / P.volume;
+      P.k0 = k0;
+      int nk = 0;
+      for (int nx = 0; nx <= nm; nx++)
+        for (int ny = (nx == 0) ? 0 : -nm; ny <= nm; ny++)
+          for (int nz = (nx == 0 && ny == 0) ? 1 : -nm; nz <= nm; nz++)
+            if (nx * nx + ny * ny + nz * nz <= nsqmax) {
+              const double kx = P.RL[0] * nx + P.RL[3] * ny + P.RL[6] * nz;
+              const double ky = P.RL[1] * nx + P.RL[4] * ny + P.RL[7] * nz;
+              const double kz = P.RL[2] * nx + P.RL[5] * ny + P.RL[8] * nz;
+              const double t = kx * kx + ky * ky + kz * kz;
+              kvec.push_back(kx); kvec.push_back(ky); kvec.push_back(kz);
+              kvec.push_back(fourpi_V * exp(-t * c) / t);
+              nk++;
+            }
+      P.nk = nk;
+      k0 += nk;
+    } else if (S.kmode == 1) {
+      MM_REQ(S.ik == 0, "P3M ik differentiation is not supported on the device (use real-space differentiation)");
+      MM_REQ(S.order >= 1 && S.order <= 7, "assignment order must be 1..7");
+      MM_REQ(radix_ok(S.nx) && radix_ok(S.ny) && radix_ok(S.nz),
+             "mesh dimensions must be <= 128 with prime factors 2,3,5,7");
+      MM_REQ(S.nalias >= 0 && S.nalias <= 7, "alias sum limit must be 0..7");
+      P.nx = S.nx; P.ny = S.ny; P.nz = S.nz; P.order = S.order;
+      for (int a = 0; a < 9; a++) P.T[a] = inv[a];
+      P.T[0] *= S.nx; P.T[3] *= S.nx; P.T[6] *= S.nx;
+      P.T[1] *= S.ny; P.T[4] *= S.ny; P.T[7] *= S.ny;
+      P.T[2] *= S.nz; P.T[5] *= S.nz; P.T[8] *= S.nz;
+      const int nzh = S.nz / 2 + 1;
+      P.mesh0 = mesh0; mesh0 += (long)S.nx * S.ny * S.nz;
+      P.cmesh0 = cmesh0; cmesh0 += (long)S.nx * S.ny * nzh;
+      P.green0 = green0;
+      green0 += (long)S.nx * S.ny * nzh;
+      const double inv_4eta2 = 1 / sq(2 * S.eta), inv_v = 1.0 / P.volume;
+      std::vector<float> gh((size_t)S.nx * S.ny * nzh);
+      for (int ix = 0; ix < S.nx; ix++) {
+        const int jx = ix > S.nx / 2 ? ix - S.nx : ix;
+        for (int iy = 0; iy < S.ny; iy++) {
+          const int jy = iy > S.ny / 2 ? iy - S.ny : iy;
+          for (int iz = 0; iz < nzh; iz++)
+            gh[((size_t)iy * nzh + iz) * S.nx + ix] =
+                (float)green_at(jx, jy, iz, S.nx, S.ny, S.nz, P.RL, inv_4eta2, inv_v, S.nalias, S.order);
+        }
+      }
+      green.insert(green.end(), gh.begin(), gh.end());
+      const int nn[3] = {S.nx, S.ny, S.nz};
+      for (int d = 0; d < 3; d++)
+        for (int t = 0; t < 128; t++) {
+          const double a = -2 * M_PI * (t % nn[d]) / nn[d];
+          twid.push_back(make_float2((float)cos(a), (float)sin(a)));
+        }
+      P.bx = largest_div_le(S.nx, 4);
+      P.by = largest_div_le(S.ny, 4);
+      P.nbx = S.nx / P.bx;
+      P.nby = S.ny / P.by;
+      P.bin0 = bin0;
+      bin0 += P.nbx * P.nby;
+    } else {
+      MM_REQ(false, "kmode must be 0 (Ewald) or 1 (P3M)");
+    }
+    P.dt = 0.001;
+    e.hsp.push_back(P);
+    e.lj_tail.push_back(S.include_lj_correction ? lj_tail(S) : 0.0);
+    e.maxg = std::max(e.maxg, S.ngroup);
+    e.maxs = std::max(e.maxs, S.nsite);
+    site0 += S.nsite;
+    group0 += S.ngroup;
+  }
+  e.nsite = site0;
+  e.ngroup = group0;
+  e.ncell_total = cell0;
+  e.nk_total = k0;
+  e.mesh_total = mesh0;
+  e.cmesh_total = cmesh0;
+  e.green_total = green0;
+  e.nbin_total = bin0;
+  e.nbcap = (maxnb + 15) / 16 * 16;
+  const int N = e.nsite, G = e.ngroup;
+  // partial-sum slots: finish blocks and x-pass blocks
+  int nb = (e.maxg + 255) / 256;
+  for (auto &P : e.hsp)
+    if (P.kmode == 1) nb = std::max(nb, (P.ny * (P.nz / 2 + 1) + 15) / 16);
+  e.nblk_part = nb;
+
+  MM_HIP(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+  MM_TRY(dalloc(&e.d_sp, e.nsys));
+  MM_TRY(dput(e.d_sp, e.hsp));
+  MM_TRY(dalloc(&e.d_gfirst, G)); MM_TRY(dput(e.d_gfirst, gfirst));
+  MM_TRY(dalloc(&e.d_gns, G)); MM_TRY(dput(e.d_gns, gns));
+  MM_TRY(dalloc(&e.d_gsys, G)); MM_TRY(dput(e.d_gsys, gsys));
+  MM_TRY(dalloc(&e.d_gexcl, G)); MM_TRY(dput(e.d_gexcl, gexcl));
+  MM_TRY(dalloc(&e.d_ginfo, G)); MM_TRY(dput(e.d_ginfo, ginfo));
+  MM_TRY(dalloc(&e.d_q, N)); MM_TRY(dput(e.d_q, q));
+  MM_TRY(dalloc(&e.d_mass, N)); MM_TRY(dput(e.d_mass, mass));
+  MM_TRY(dalloc(&e.d_r, 3 * (size_t)N));
+  MM_TRY(dalloc(&e.d_v, 3 * (size_t)N));
+  MM_TRY(dalloc(&e.d_f, 3 * (size_t)N));
+  MM_TRY(dalloc(&e.d_phi, (size_t)N));
+  MM_TRY(dalloc(&e.d_gcell, G));
+  MM_TRY(dalloc(&e.d_cell_count, e.ncell_total + 1));
+  MM_TRY(dalloc(&e.d_cell_start, e.ncell_total + 1));
+  MM_TRY(dalloc(&e.d_cell_fill, e.ncell_total + 1));
+  MM_TRY(dalloc(&e.d_order, G));
+  MM_TRY(dalloc(&e.d_last, 3 * (size_t)G));
+  MM_TRY(dalloc(&e.d_sten, sten.size())); MM_TRY(dput(e.d_sten, sten));
+  MM_TRY(dalloc(&e.d_nbr, (size_t)G * e.nbcap));
+  MM_TRY(dalloc(&e.d_nnb, G));
+  MM_TRY(dalloc(&e.d_rebuild, e.nsys));
+  MM_TRY(dalloc(&e.d_err, 1));
+  MM_TRY(dalloc(&e.d_gpos, G));
+  MM_TRY(dalloc(&e.d_spk, (size_t)G * MAXS));
+  MM_TRY(dalloc(&e.d_gmeta, G));
+  MM_TRY(dalloc(&e.d_pf, (size_t)G * MAXS * 4));
+  MM_TRY(dalloc(&e.d_plj, (size_t)G * MAXS * 3));
+  MM_TRY(dalloc(&e.d_gacc, (size_t)G * 10));
+  MM_TRY(dalloc(&e.d_tab, tab.size())); MM_TRY(dput(e.d_tab, tab));
+  MM_TRY(dalloc(&e.d_ljt, ljt.size())); MM_TRY(dput(e.d_ljt, ljt));
+  MM_TRY(dalloc(&e.d_kres, 4 * (size_t)N + 9 * e.nsys));
+  MM_TRY(dalloc(&e.d_kvec, kvec.size())); MM_TRY(dput(e.d_kvec, kvec));
+  MM_TRY(dalloc(&e.d_ksc, 2 * (size_t)e.nk_total));
+  MM_TRY(dalloc(&e.d_mesh, e.mesh_total));
+  MM_TRY(dalloc(&e.d_cmesh, e.cmesh_total));
+  MM_TRY(dalloc(&e.d_green, green.size())); MM_TRY(dput(e.d_green, green));
+  MM_TRY(dalloc(&e.d_twid, twid.size())); MM_TRY(dput(e.d_twid, twid));
+  MM_TRY(dalloc(&e.d_sgrid, N));
+  MM_TRY(dalloc(&e.d_sfrac, N));
+  MM_TRY(dalloc(&e.d_bin_count, e.nbin_total + 1));
+  MM_TRY(dalloc(&e.d_bin_start, e.nbin_total + 1));
+  MM_TRY(dalloc(&e.d_bin_fill, e.nbin_total + 1));
+  MM_TRY(dalloc(&e.d_bin_sites, N));
+  MM_TRY(dalloc(&e.d_acc, (size_t)e.nsys * NACC));
+  MM_TRY(dalloc(&e.d_part, (size_t)e.nsys * e.nblk_part * 16));
+  MM_TRY(dalloc(&e.d_done, 4 * (size_t)e.nsys));
+  {
+    std::vector<int> cs(e.ncell_total + 1, 0);
+    cs[e.ncell_total] = G;
+    MM_HIP(hipMemcpy(e.d_cell_start + e.ncell_total, &cs[e.ncell_total], sizeof(int),
+                     hipMemcpyHostToDevice));
+  }
+  MM_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+void engine_destroy(Engine &e) {
+  if (e.gexec) hipGraphExecDestroy(e.gexec);
+  if (e.graph) hipGraphDestroy(e.graph);
+  void *ptrs[] = {e.d_sp, e.d_gfirst, e.d_gns, e.d_gsys, e.d_gexcl, e.d_ginfo, e.d_q, e.d_mass,
+                  e.d_r, e.d_v, e.d_f, e.d_r0, e.d_gcell, e.d_cell_count, e.d_cell_start,
+                  e.d_cell_fill, e.d_order, e.d_last, e.d_sten, e.d_nbr, e.d_nnb, e.d_rebuild,
+                  e.d_err, e.d_gpos, e.d_spk, e.d_gmeta, e.d_pf, e.d_plj, e.d_gacc, e.d_tab,
+                  e.d_ljt, e.d_kres, e.d_kvec, e.d_ksc, e.d_mesh, e.d_cmesh, e.d_green, e.d_twid,
+                  e.d_sgrid, e.d_sfrac, e.d_bin_count, e.d_bin_start, e.d_bin_fill,
+                  e.d_bin_sites, e.d_acc, e.d_part, e.d_done, e.d_phi, e.d_scratch};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (e.stream) hipStreamDestroy(e.stream);
+  e = Engine();
+}
+
+}  // namespace mmamd

@@ -1,0 +1,147 @@
+// engine.hpp -- host-side engine object: owns every device buffer of a batch
+// of periodic cubic systems and sequences the kernels of one force
+// evaluation / one MD step on a single HIP stream.
+#pragma once
+#include "mm_common.hpp"
+#include <vector>
+#include <string>
+
+namespace mmamd {
+
+#define MM_HIP(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      mmamd::set_last_error(std::string("HIP error ") + hipGetErrorString(e_) + \
+                            " at " __FILE__ ":" + std::to_string(__LINE__));   \
+      return -2;                                                               \
+    }                                                                          \
+  } while (0)
+
+void set_last_error(const std::string &s);
+const char *last_error();
+
+// accumulator slots per system (FP64)
+enum : int {
+  ACC_ELEC = 0,   // 1/2 q.phi
+  ACC_LJ = 1,     // pair LJ (smoothed), without tail
+  ACC_KIN = 2,    // kinetic energy
+  ACC_VIR = 3,    // 9 virial components (rows x, y, z)
+  ACC_KSPACE_E = 12, // reciprocal-space energy from the k-sum (diagnostic)
+  ACC_TOP1 = 13,  // largest first-site displacement since the last NL build
+  ACC_TOP2 = 14,  // second largest
+  NACC = 16
+};
+
+struct SystemInput {       // mirrors mm_system_desc of include/molmodel_amd.h
+  int nsite, ngroup;
+  std::vector<int> gfirst, gns;
+  std::vector<double> q, sig, eps, mass;
+  std::vector<int> mask;
+  std::vector<int> excl;   // pairs
+  double boxl, rc, width, skin, eta;
+  int kmode; double kc;
+  int nx, ny, nz, order, nalias, ik;
+  int include_lj_correction;
+  int geometric_sigma;
+};
+
+struct Engine {
+  int nsys = 0, nsite = 0, ngroup = 0;
+  int maxg = 0;               // max groups of any system (grid x extent)
+  int maxs = 0;
+  std::vector<SystemInput> in;
+  std::vector<SysParams> hsp;
+  std::vector<double> lj_tail;   // per system (host FP64)
+  hipStream_t stream = nullptr;
+
+  SysParams *d_sp = nullptr;
+  // topology, original order
+  int *d_gfirst = nullptr, *d_gns = nullptr, *d_gsys = nullptr;
+  int *d_gexcl = nullptr;      // per group: bit (k*MAXS+l) set if sites k,l excluded
+  int *d_ginfo = nullptr;      // per group: packed per-site (mask | type<<2) bytes
+  double *d_q = nullptr, *d_mass = nullptr;
+  int *d_stype = nullptr, *d_smask = nullptr;
+  // state, FP64 SoA (3 arrays of nsite each, contiguous: x..., y..., z...)
+  double *d_r = nullptr, *d_v = nullptr, *d_f = nullptr, *d_r0 = nullptr;
+  // neighbor list
+  int ncell_total = 0;
+  int *d_gcell = nullptr, *d_cell_count = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr;
+  int *d_order = nullptr;     // sorted gs -> original group
+  double *d_last = nullptr;   // 3*ngroup first-site positions at the last build
+  int4 *d_sten = nullptr;
+  int nbcap = 0;              // neighbor slots per group
+  int *d_nbr = nullptr, *d_nnb = nullptr;
+  int *d_rebuild = nullptr;   // per system flag
+  int *d_err = nullptr;
+  long long *d_nbuilds = nullptr;
+  // packed (sorted order)
+  double4 *d_gpos = nullptr;
+  float4 *d_spk = nullptr;    // MAXS per group: offset xyz + charge
+  int2 *d_gmeta = nullptr;    // (nsites, packed masks/types)
+  // pair outputs (sorted slots)
+  double *d_pf = nullptr;     // 4 per slot: fx fy fz phi (force mode) or ex ey ez phi
+  double *d_plj = nullptr;    // 3 per slot: LJ + smoothing force (field mode)
+  double *d_gacc = nullptr;   // per sorted group: u_lj, 9 virial
+  // tables
+  float4 *d_tab = nullptr;
+  float2 *d_ljt = nullptr;
+  // k-space / P3M per-site results (original order): phi, ex, ey, ez
+  double *d_kres = nullptr;
+  // Ewald
+  double *d_kvec = nullptr;   // 4 per k: kx ky kz a_k
+  double *d_ksc = nullptr;    // 2 per k: S, C
+  int nk_total = 0;
+  // P3M
+  long mesh_total = 0, cmesh_total = 0, green_total = 0;
+  float *d_mesh = nullptr;
+  float2 *d_cmesh = nullptr;
+  float *d_green = nullptr;
+  float2 *d_twid = nullptr;   // per system: twiddles for nx, ny, nz (3*128 entries)
+  int4 *d_sgrid = nullptr;    // per site: jx jy jz bin
+  float4 *d_sfrac = nullptr;  // per site: sx sy sz q
+  int nbin_total = 0;
+  int *d_bin_count = nullptr, *d_bin_start = nullptr, *d_bin_fill = nullptr, *d_bin_sites = nullptr;
+  // reductions
+  double *d_acc = nullptr;    // NACC per system
+  double *d_part = nullptr;   // per-block partials
+  int *d_done = nullptr;      // per-system block counters
+  int nblk_finish = 0;         // finish blocks per system (max)
+  int nblk_part = 0;           // partial-sum slots per system
+  // MD
+  double bond = 1.0, angle = 109.47, tol = 1e-8;
+  int md_ready = 0;
+  int first_eval = 1;
+  int md_phase = 0;          // 0: r, v, F at the same step; 1: drifted half step ahead
+  double *d_phi = nullptr;   // per-site total potential of the last evaluation
+  double *d_scratch = nullptr; size_t scratch_n = 0;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_mode = -1;
+};
+
+int engine_create(Engine &e, const std::vector<SystemInput> &in);
+void engine_destroy(Engine &e);
+
+// launchers (each returns 0 or a negative status)
+int launch_nl_build(Engine &e, int force);
+int launch_pack(Engine &e);
+int launch_pair(Engine &e, int field_mode, int want_virial);
+int launch_kspace(Engine &e, int want_virial);
+int launch_p3m(Engine &e, int want_virial);
+int launch_finish(Engine &e, int mode);
+int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out);
+int launch_field_out(Engine &e, double *out, double *ulj);
+int launch_skin(Engine &e);
+int launch_set_rebuild(Engine &e);
+int launch_constrain_initial(Engine &e);
+int launch_start_md(Engine &e);
+
+enum : int {   // launch_finish modes
+  FIN_FORCES = 0,        // forces + energies only
+  FIN_KICK = 1,          // + v += F dt/2m, RATTLE velocities, kinetic energy
+  FIN_DRIFT = 2,         // + v += F dt/2m, r += v dt, SHAKE positions
+};
+
+}  // namespace mmamd

@@ -1,0 +1,249 @@
+// nbrlist.hip -- device cell neighbor list over groups.
+//
+// Reproduces the pair set of NeighborList::rebuild_periodic
+// (nbrlist.cpp:222-251) bit for bit:
+//   * cell of a group = cell_index(floor(x/lc + 0.5), ...) of its first site,
+//     FP64, lc = L*(1.0/ncell) (nbrlist.cpp:97, 228-231; pbc.h:25-31);
+//   * same-cell pairs are always listed (nbrlist.cpp:243-244);
+//   * pairs in distinct neighbouring cells are listed iff the FP64
+//     minimum-image distance^2 < (rc+skin)^2 (nbrlist.cpp:245-248,
+//     bcond.hpp:56-60, pbc.h:8-14), evaluated in the order the reference
+//     evaluates it: first argument in the cell that owns the cell pair row
+//     (pbin.c:6), no FMA contraction.
+// Layout: groups are counting-sorted by cell (deterministic order within a
+// cell), and each sorted group gets a full list (both directions) of sorted
+// neighbour indices in a fixed-capacity row, for the pair kernel.
+#include "engine.hpp"
+#include "scan.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mmamd {
+
+__device__ __forceinline__ double min_image_sq(double ax, double ay, double az, double bx,
+                                               double by, double bz, double L, double invL) {
+#pragma clang fp contract(off)
+  double x = ax - bx, y = ay - by, z = az - bz;
+  x -= L * floor(x * invL + 0.5);
+  y -= L * floor(y * invL + 0.5);
+  z -= L * floor(z * invL + 0.5);
+  return x * x + y * y + z * z;
+}
+
+__global__ void k_nl_assign(const SysParams *__restrict__ sp, const int *__restrict__ gfirst,
+                            const double *__restrict__ r, int nsite_total,
+                            const int *__restrict__ rebuild, int *__restrict__ gcell,
+                            double *__restrict__ last, int *__restrict__ cell_count) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y;
+  if (!rebuild[s]) return;
+  const SysParams &P = sp[s];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.ngroup) return;
+  const int g = P.group0 + i;
+  const int site = gfirst[g];
+  const double x = r[site], y = r[nsite_total + site], z = r[2 * nsite_total + site];
+  const int n = P.ncell;
+  const int ix = (int)floor(x / P.lc + 0.5);
+  const int iy = (int)floor(y / P.lc + 0.5);
+  const int iz = (int)floor(z / P.lc + 0.5);
+  const int c = P.cell0 + (mod_nonneg(ix, n) * n + mod_nonneg(iy, n)) * n + mod_nonneg(iz, n);
+  gcell[g] = c;
+  last[3 * g] = x;
+  last[3 * g + 1] = y;
+  last[3 * g + 2] = z;
+  atomicAdd(&cell_count[c], 1);
+}
+
+// one block per system: exclusive scan of that system's cells into sorted
+// group offsets; counts and fill cursors are cleared for the next build
+__global__ void k_nl_scan(const SysParams *__restrict__ sp, const int *__restrict__ rebuild,
+                          int *__restrict__ cell_count, int *__restrict__ cell_start,
+                          int *__restrict__ cell_fill) {
+  const int s = blockIdx.y;
+  if (!rebuild[s]) return;
+  const SysParams &P = sp[s];
+  block_scan_clear(cell_count + P.cell0, cell_start + P.cell0, cell_fill + P.cell0,
+                   P.ncell * P.ncell * P.ncell, P.group0);
+}
+
+__global__ void k_nl_scatter(const SysParams *__restrict__ sp, const int *__restrict__ rebuild,
+                             const int *__restrict__ gcell, const int *__restrict__ cell_start,
+                             int *__restrict__ cell_fill, int *__restrict__ order) {
+  const int s = blockIdx.y;
+  if (!rebuild[s]) return;
+  const SysParams &P = sp[s];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.ngroup) return;
+  const int g = P.group0 + i;
+  const int c = gcell[g];
+  order[cell_start[c] + atomicAdd(&cell_fill[c], 1)] = g;
+}
+
+// ascending group id within each cell, so the sorted order is deterministic
+__global__ void k_nl_sortcell(const SysParams *__restrict__ sp, const int *__restrict__ rebuild,
+                              const int *__restrict__ cell_start, int *__restrict__ order) {
+  const int s = blockIdx.y;
+  if (!rebuild[s]) return;
+  const SysParams &P = sp[s];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.ncell * P.ncell * P.ncell) return;
+  const int b = cell_start[P.cell0 + c], e = cell_start[P.cell0 + c + 1];
+  for (int i = b + 1; i < e; i++) {
+    const int v = order[i];
+    int j = i - 1;
+    while (j >= b && order[j] > v) { order[j + 1] = order[j]; j--; }
+    order[j + 1] = v;
+  }
+}
+
+// one wavefront per sorted group: walk the stencil cells, test candidates,
+// compact the accepted neighbours with a ballot
+__global__ void __launch_bounds__(256) k_nl_pairs(
+    const SysParams *__restrict__ sp, const int *__restrict__ rebuild,
+    const int *__restrict__ order, const int *__restrict__ gcell,
+    const int *__restrict__ cell_start, const double *__restrict__ last,
+    const int4 *__restrict__ sten, int nbcap, int *__restrict__ nbr, int *__restrict__ nnb,
+    int *__restrict__ err) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y;
+  if (!rebuild[s]) return;
+  const SysParams &P = sp[s];
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= P.ngroup) return;
+  const int gs = P.group0 + wave;
+  const int g = order[gs];
+  const int n = P.ncell;
+  const int A = gcell[g] - P.cell0;
+  const int ax = A / (n * n), ay = (A / n) % n, az = A % n;
+  const double gx = last[3 * g], gy = last[3 * g + 1], gz = last[3 * g + 2];
+  const double L = P.L, invL = 1 / P.L;
+  const double c2 = P.nl_c2;
+  int cnt = 0;
+  int *row = nbr + (long)gs * nbcap;
+  for (int t0 = 0; t0 < P.nsten; t0 += 64) {
+    const int t = t0 + lane;
+    int bstart = 0, nb = 0, B = -1;
+    if (t < P.nsten) {
+      const int4 o = sten[P.sten0 + t];
+      const int bxx = (ax + o.x) % n, byy = (ay + o.y) % n, bzz = (az + o.z) % n;
+      B = (bxx * n + byy) * n + bzz;
+      bstart = cell_start[P.cell0 + B];
+      nb = cell_start[P.cell0 + B + 1] - bstart;
+    }
+    // the cell that owns the (A,B) row supplies the first argument
+    int ra = A, rb = B;
+    if (B >= 0) canonical_pair(ra, rb);
+    const bool g_first = (ra == A);
+    for (int m = 0; __any(m < nb); m++) {
+      bool take = false;
+      int hs = -1;
+      if (m < nb) {
+        hs = bstart + m;
+        if (hs != gs) {
+          if (B == A) {
+            take = true;
+          } else {
+            const int h = order[hs];
+            const double hx = last[3 * h], hy = last[3 * h + 1], hz = last[3 * h + 2];
+            const double d2 = g_first ? min_image_sq(gx, gy, gz, hx, hy, hz, L, invL)
+                                      : min_image_sq(hx, hy, hz, gx, gy, gz, L, invL);
+            take = d2 < c2;
+          }
+        }
+      }
+      const unsigned long long bal = __ballot(take);
+      if (take) {
+        const int pos = cnt + __popcll(bal & ((1ull << lane) - 1ull));
+        if (pos < nbcap) row[pos] = hs;
+      }
+      cnt += __popcll(bal);
+    }
+  }
+  if (lane == 0) {
+    if (cnt > nbcap) {
+      atomicOr(err, ERR_NL_OVERFLOW);
+      cnt = nbcap;
+    }
+    nnb[gs] = cnt;
+  }
+}
+
+__global__ void k_set_flags(int *f, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = v;
+}
+
+// skin test for externally supplied positions (nbrlist.cpp:64-76): one
+// block per system; a rebuild already requested stays requested
+__global__ void k_skin(const SysParams *__restrict__ sp, const int *__restrict__ gfirst,
+                       const double *__restrict__ r, int N, const double *__restrict__ last,
+                       int *__restrict__ rebuild) {
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  double t1 = 0, t2 = 0;
+  for (int i = threadIdx.x; i < P.ngroup; i += blockDim.x) {
+    const int g = P.group0 + i, st = gfirst[g];
+    const double dx = r[st] - last[3 * g], dy = r[N + st] - last[3 * g + 1],
+                 dz = r[2 * N + st] - last[3 * g + 2];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (d > t1) { t2 = t1; t1 = d; } else if (d > t2) t2 = d;
+  }
+  __shared__ double a1[16], a2[16];
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double b1 = __shfl_xor(t1, o), b2 = __shfl_xor(t2, o);
+    if (b1 > t1) { t2 = fmax(t1, b2); t1 = b1; } else if (b1 > t2) t2 = b1;
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { a1[w] = t1; a2[w] = t2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double u1 = 0, u2 = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+      if (a1[k] > u1) { u2 = fmax(u1, a2[k]); u1 = a1[k]; } else if (a1[k] > u2) u2 = a1[k];
+    }
+    if (u1 + u2 > P.skin) rebuild[s] = 1;
+  }
+}
+
+int launch_skin(Engine &e) {
+  hipLaunchKernelGGL(k_skin, dim3(1, e.nsys), dim3(1024), 0, e.stream, e.d_sp, e.d_gfirst, e.d_r,
+                     e.nsite, e.d_last, e.d_rebuild);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_set_rebuild(Engine &e) {
+  hipLaunchKernelGGL(k_set_flags, dim3((e.nsys + 63) / 64), dim3(64), 0, e.stream, e.d_rebuild,
+                     e.nsys, 1);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_nl_build(Engine &e, int force) {
+  if (force) {
+    int rc = launch_set_rebuild(e);
+    if (rc) return rc;
+  }
+  const int tb = 256;
+  dim3 gg((e.maxg + tb - 1) / tb, e.nsys);
+  const double *r = e.d_r;
+  hipLaunchKernelGGL(k_nl_assign, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_gfirst, r, e.nsite,
+                     e.d_rebuild, e.d_gcell, e.d_last, e.d_cell_count);
+  hipLaunchKernelGGL(k_nl_scan, dim3(1, e.nsys), dim3(1024), 0, e.stream, e.d_sp, e.d_rebuild,
+                     e.d_cell_count, e.d_cell_start, e.d_cell_fill);
+  hipLaunchKernelGGL(k_nl_scatter, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_rebuild, e.d_gcell,
+                     e.d_cell_start, e.d_cell_fill, e.d_order);
+  int maxc = 0;
+  for (auto &p : e.hsp) maxc = std::max(maxc, p.ncell * p.ncell * p.ncell);
+  hipLaunchKernelGGL(k_nl_sortcell, dim3((maxc + tb - 1) / tb, e.nsys), dim3(tb), 0, e.stream,
+                     e.d_sp, e.d_rebuild, e.d_cell_start, e.d_order);
+  hipLaunchKernelGGL(k_nl_pairs, dim3((e.maxg + 3) / 4, e.nsys), dim3(256), 0, e.stream, e.d_sp,
+                     e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start, e.d_last, e.d_sten,
+                     e.nbcap, e.d_nbr, e.d_nnb, e.d_err);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mmamd

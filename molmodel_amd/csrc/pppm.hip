@@ -1,0 +1,504 @@
+// pppm.hip -- P3M on the device (pppm.c:498-641).
+//
+// Steps, each its own kernel on one stream:
+//   grid     s = diag(n) L^-1 r in FP64, nearest/lower mesh point and the
+//            fractional offset as the reference forms them (pppm.c:523-548);
+//            each site is counted into the mesh column block that holds its
+//            base point;
+//   scan / scatter   counting sort of sites into column-block bins;
+//   spread   one workgroup per column block (bx x by x nz points) pulls the
+//            sites of its own and neighbouring bins and accumulates
+//            q w_x w_y w_z into the points it owns in LDS as 64-bit fixed
+//            point (exact, so the mesh is independent of visiting order),
+//            then writes them once; no global atomics;
+//   fft      Stockham mixed-radix (4,2,3,5,7) line FFTs in LDS: z (real to
+//            half spectrum), y, then x forward, multiply by the influence
+//            function (and form the mesh energy/virial), x inverse, y
+//            inverse, z (half spectrum to real), unnormalised like FFTW;
+//   gather   phi_i = sum w phi_m and, with real-space differentiation (the
+//            reference default), E_i = (diag(n) L^-1)^T (sum -dw phi_m)
+//            (pppm.c:603-640).
+// Assignment weights are the cardinal B-splines that the reference's
+// polynomial tables tabulate (pppm.c:69-275), evaluated by recursion in FP32.
+#include "engine.hpp"
+#include "scan.hpp"
+
+namespace mmamd {
+
+constexpr float FIX_SCALE = 4294967296.0f;   // 2^32
+constexpr double FIX_INV = 1.0 / 4294967296.0;
+
+__device__ __forceinline__ void bspline_w(float s, int order, float *w, float *ndw) {
+  // w[k] = M_p(u + p - 1 - k), u = s + 1/2; ndw[k] = -dw/ds
+  const float u = s + 0.5f;
+  float m[MAXORDER + 1];
+#pragma unroll
+  for (int k = 0; k <= MAXORDER; k++) m[k] = 0.f;
+  m[0] = 1.f;
+  for (int p = 2; p <= order; p++) {
+    if (p == order && ndw) {
+      for (int k = 0; k < order; k++) {
+        const float a = k < p - 1 ? m[k] : 0.f;
+        const float b = k >= 1 ? m[k - 1] : 0.f;
+        ndw[order - 1 - k] = b - a;
+      }
+    }
+    const float inv = 1.f / (float)(p - 1);
+    for (int k = p - 1; k >= 0; k--) {
+      const float x = u + (float)k;
+      const float a = k < p - 1 ? m[k] : 0.f;
+      const float b = k >= 1 ? m[k - 1] : 0.f;
+      m[k] = (x * a + ((float)p - x) * b) * inv;
+    }
+  }
+  if (order == 1 && ndw) ndw[0] = 0.f;
+  for (int k = 0; k < order; k++) w[order - 1 - k] = m[k];
+}
+
+__global__ void k_p3m_grid(const SysParams *__restrict__ sp, const double *__restrict__ r,
+                           const double *__restrict__ q, int N, int4 *__restrict__ sgrid,
+                           float4 *__restrict__ sfrac, int *__restrict__ bin_count) {
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  if (P.kmode != 1) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nsite) return;
+  const int si = P.site0 + i;
+  const double x = r[si], y = r[N + si], z = r[2 * N + si];
+  const double *T = P.T;
+  double sx = T[0] * x + T[3] * y + T[6] * z;
+  double sy = T[1] * x + T[4] * y + T[7] * z;
+  double sz = T[2] * x + T[5] * y + T[8] * z;
+  int jx, jy, jz;
+  if (P.order & 1) {
+    jx = (int)floor(sx + 0.5); jy = (int)floor(sy + 0.5); jz = (int)floor(sz + 0.5);
+    sx -= jx; sy -= jy; sz -= jz;
+  } else {
+    jx = (int)floor(sx); jy = (int)floor(sy); jz = (int)floor(sz);
+    sx -= jx + 0.5; sy -= jy + 0.5; sz -= jz + 0.5;
+  }
+  jx = mod_nonneg(jx, P.nx); jy = mod_nonneg(jy, P.ny); jz = mod_nonneg(jz, P.nz);
+  const int bin = P.bin0 + (jx / P.bx) * P.nby + jy / P.by;
+  sgrid[si] = make_int4(jx, jy, jz, bin);
+  sfrac[si] = make_float4((float)sx, (float)sy, (float)sz, (float)q[si]);
+  atomicAdd(&bin_count[bin], 1);
+}
+
+__global__ void k_p3m_binscan(const SysParams *__restrict__ sp, int *__restrict__ bin_count,
+                              int *__restrict__ bin_start, int *__restrict__ bin_fill) {
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  if (P.kmode != 1) return;
+  block_scan_clear(bin_count + P.bin0, bin_start + P.bin0, bin_fill + P.bin0, P.nbx * P.nby,
+                   P.site0);
+}
+
+__global__ void k_p3m_binscatter(const SysParams *__restrict__ sp, const int4 *__restrict__ sgrid,
+                                 const int *__restrict__ bin_start, int *__restrict__ bin_fill,
+                                 int *__restrict__ bin_sites) {
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  if (P.kmode != 1) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nsite) return;
+  const int si = P.site0 + i;
+  const int b = sgrid[si].w;
+  bin_sites[bin_start[b] + atomicAdd(&bin_fill[b], 1)] = si;
+}
+
+__device__ __forceinline__ int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+constexpr int SPREAD_MAXPTS = 4 * 4 * 128;
+
+__global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict__ sp,
+                                                    const int4 *__restrict__ sgrid,
+                                                    const float4 *__restrict__ sfrac,
+                                                    const int *__restrict__ bin_start,
+                                                    const int *__restrict__ bin_sites,
+                                                    float *__restrict__ mesh) {
+  __shared__ unsigned long long acc[SPREAD_MAXPTS];
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  if (P.kmode != 1) return;
+  const int blk = blockIdx.x;
+  if (blk >= P.nbx * P.nby) return;
+  const int bx = P.bx, by = P.by, nx = P.nx, ny = P.ny, nz = P.nz, p = P.order;
+  const int bi = blk / P.nby, bj = blk % P.nby;
+  const int X0 = bi * bx, Y0 = bj * by;
+  const int npts = bx * by * nz;
+  for (int t = threadIdx.x; t < npts; t += blockDim.x) acc[t] = 0ull;
+  __syncthreads();
+  const int istart = -((p + 1) / 2 - 1);
+  // neighbouring column blocks whose sites reach the owned columns
+  int xlo = floor_div(X0 - istart - p + 1, bx), xhi = floor_div(X0 + bx - 1 - istart, bx);
+  int ylo = floor_div(Y0 - istart - p + 1, by), yhi = floor_div(Y0 + by - 1 - istart, by);
+  if (xhi - xlo + 1 >= P.nbx) { xlo = 0; xhi = P.nbx - 1; }
+  if (yhi - ylo + 1 >= P.nby) { ylo = 0; yhi = P.nby - 1; }
+  for (int cx = xlo; cx <= xhi; cx++) {
+    const int ccx = mod_nonneg(cx, P.nbx);
+    for (int cy = ylo; cy <= yhi; cy++) {
+      const int ccy = mod_nonneg(cy, P.nby);
+      const int b = P.bin0 + ccx * P.nby + ccy;
+      const int b0 = bin_start[b], b1 = bin_start[b + 1];
+      for (int t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
+        const int si = bin_sites[t];
+        const int4 g = sgrid[si];
+        const float4 f = sfrac[si];
+        float wx[MAXORDER], wy[MAXORDER], wz[MAXORDER];
+        bspline_w(f.x, p, wx, nullptr);
+        bspline_w(f.y, p, wy, nullptr);
+        bspline_w(f.z, p, wz, nullptr);
+        for (int a = 0; a < p; a++) {
+          const int gx = mod_nonneg(g.x + istart + a, nx) - X0;
+          if (gx < 0 || gx >= bx) continue;
+          for (int bb = 0; bb < p; bb++) {
+            const int gy = mod_nonneg(g.y + istart + bb, ny) - Y0;
+            if (gy < 0 || gy >= by) continue;
+            const float qw = f.w * wx[a] * wy[bb];
+            unsigned long long *col = acc + (gx * by + gy) * nz;
+            for (int c = 0; c < p; c++) {
+              const int gz = mod_nonneg(g.z + istart + c, nz);
+              const long long v = __float2ll_rn(qw * wz[c] * FIX_SCALE);
+              atomicAdd(col + gz, (unsigned long long)v);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < npts; t += blockDim.x) {
+    const int gz = t % nz, gy = (t / nz) % by, gx = t / (nz * by);
+    mesh[P.mesh0 + ((long)(X0 + gx) * ny + (Y0 + gy)) * nz + gz] =
+        (float)((double)(long long)acc[t] * FIX_INV);
+  }
+}
+
+// ---------------- line FFTs in LDS ----------------
+constexpr int FFT_TB = 256;
+constexpr int FFT_T = 16;                 // threads per line
+constexpr int FFT_LINES = FFT_TB / FFT_T; // lines per block
+constexpr int FFT_MAXN = 128;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+__device__ __forceinline__ int fft_radices(int n, int *rad) {
+  int k = 0;
+  while (n % 4 == 0) { rad[k++] = 4; n /= 4; }
+  while (n % 2 == 0) { rad[k++] = 2; n /= 2; }
+  while (n % 3 == 0) { rad[k++] = 3; n /= 3; }
+  while (n % 5 == 0) { rad[k++] = 5; n /= 5; }
+  while (n % 7 == 0) { rad[k++] = 7; n /= 7; }
+  return k;
+}
+
+// twiddle exp(sign*2*pi*i*t/n) from tw[t] = exp(-2 pi i t/n)
+__device__ __forceinline__ float2 twid(const float2 *tw, int t, int n, int sign) {
+  const float2 w = tw[t % n];
+  return sign < 0 ? w : make_float2(w.x, -w.y);
+}
+
+// Stockham autosort FFT of FFT_LINES lines of length n held in x (line l at
+// x + l*FFT_MAXN); every thread of the block must call it.  Returns the
+// buffer holding the result.
+__device__ float2 *lds_fft_lines(float2 *x, float2 *y, int n, const float2 *tw, int sign) {
+  int rad[16];
+  const int nr = fft_radices(n, rad);
+  const int l = threadIdx.x / FFT_T, tid = threadIdx.x % FFT_T;
+  int p = 1;
+  for (int st = 0; st < nr; st++) {
+    const int R = rad[st];
+    const int nb = n / R;
+    const int tstep = n / (p * R);
+    float2 *xl = x + l * FFT_MAXN, *yl = y + l * FFT_MAXN;
+    for (int i = tid; i < nb; i += FFT_T) {
+      const int k = i % p;
+      float2 u[7], v[7];
+      for (int r = 0; r < R; r++) {
+        u[r] = xl[i + r * nb];
+        if (r > 0 && k > 0) u[r] = cmul(u[r], twid(tw, r * k * tstep, n, sign));
+      }
+      if (R == 2) {
+        v[0] = cadd(u[0], u[1]); v[1] = csub(u[0], u[1]);
+      } else if (R == 4) {
+        const float2 a = cadd(u[0], u[2]), b = csub(u[0], u[2]);
+        const float2 c = cadd(u[1], u[3]), d = csub(u[1], u[3]);
+        // multiply d by sign*i
+        const float2 di = sign < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+        v[0] = cadd(a, c); v[2] = csub(a, c);
+        v[1] = cadd(b, di); v[3] = csub(b, di);
+      } else {
+        const int ws = n / R;
+        for (int r = 0; r < R; r++) {
+          float2 acc = u[0];
+          for (int t = 1; t < R; t++) acc = cadd(acc, cmul(u[t], twid(tw, ((r * t) % R) * ws, n, sign)));
+          v[r] = acc;
+        }
+      }
+      const int j = (i - k) * R + k;
+      for (int r = 0; r < R; r++) yl[j + r * p] = v[r];
+    }
+    __syncthreads();
+    float2 *t = x; x = y; y = t;
+    p *= R;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void load_tw(float2 *stw, const float2 *tw, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
+}
+
+// pass 0: z lines, real -> half spectrum.  pass 1/3: y lines fwd/inv.
+// pass 4: z lines, half spectrum -> real.
+__global__ void __launch_bounds__(FFT_TB) k_fft_pass(const SysParams *__restrict__ sp,
+                                                     const float2 *__restrict__ twid_all,
+                                                     float *__restrict__ mesh,
+                                                     float2 *__restrict__ cmesh, int pass) {
+  __shared__ float2 bx_[FFT_LINES * FFT_MAXN], by_[FFT_LINES * FFT_MAXN];
+  __shared__ float2 stw[FFT_MAXN];
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  if (P.kmode != 1) return;
+  const int nx = P.nx, ny = P.ny, nz = P.nz, nzh = nz / 2 + 1;
+  int nlines, n, twoff;
+  if (pass == 0 || pass == 4) { nlines = nx * ny; n = nz; twoff = 2 * FFT_MAXN; }
+  else { nlines = nx * nzh; n = ny; twoff = FFT_MAXN; }
+  if ((int)(blockIdx.x * FFT_LINES) >= nlines) return;
+  load_tw(stw, twid_all + (long)s * 3 * FFT_MAXN + twoff, n);
+  const int l = threadIdx.x / FFT_T, tid = threadIdx.x % FFT_T;
+  const int line = blockIdx.x * FFT_LINES + l;
+  const bool on = line < nlines;
+  float2 *xl = bx_ + l * FFT_MAXN;
+  const int sign = (pass == 3 || pass == 4) ? 1 : -1;
+  // load
+  if (pass == 0) {
+    if (on) {
+      const float *src = mesh + P.mesh0 + (long)line * nz;
+      for (int t = tid; t < n; t += FFT_T) xl[t] = make_float2(src[t], 0.f);
+    }
+  } else if (pass == 4) {
+    if (on) {
+      const float2 *src = cmesh + P.cmesh0 + (long)line * nzh;
+      for (int t = tid; t < nzh; t += FFT_T) xl[t] = src[t];
+    }
+    __syncthreads();
+    if (on)
+      for (int t = nzh + tid; t < n; t += FFT_T) {
+        const float2 c = xl[n - t];
+        xl[t] = make_float2(c.x, -c.y);
+      }
+  } else {
+    if (on) {
+      const int xi = line / nzh, kz = line % nzh;
+      const float2 *src = cmesh + P.cmesh0 + (long)xi * ny * nzh + kz;
+      for (int t = tid; t < n; t += FFT_T) xl[t] = src[(long)t * nzh];
+    }
+  }
+  __syncthreads();
+  float2 *res = lds_fft_lines(bx_, by_, n, stw, sign);
+  float2 *rl = res + l * FFT_MAXN;
+  if (!on) return;
+  if (pass == 0) {
+    float2 *dst = cmesh + P.cmesh0 + (long)line * nzh;
+    for (int t = tid; t < nzh; t += FFT_T) dst[t] = rl[t];
+  } else if (pass == 4) {
+    float *dst = mesh + P.mesh0 + (long)line * nz;
+    for (int t = tid; t < n; t += FFT_T) dst[t] = rl[t].x;
+  } else {
+    const int xi = line / nzh, kz = line % nzh;
+    float2 *dst = cmesh + P.cmesh0 + (long)xi * ny * nzh + kz;
+    for (int t = tid; t < n; t += FFT_T) dst[(long)t * nzh] = rl[t];
+  }
+}
+
+// x lines: forward FFT, multiply by the influence function, inverse FFT.
+// Also the mesh energy 1/2 sum G |rho_k|^2 and its virial
+// W_ab = sum e_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 eta^2))).
+__global__ void __launch_bounds__(FFT_TB) k_fft_xgreen(const SysParams *__restrict__ sp,
+                                                       const float2 *__restrict__ twid_all,
+                                                       const float *__restrict__ green,
+                                                       float2 *__restrict__ cmesh,
+                                                       double *__restrict__ part,
+                                                       int *__restrict__ done,
+                                                       double *__restrict__ acc,
+                                                       double *__restrict__ kvir, int want_virial,
+                                                       int maxblk) {
+  __shared__ float2 bx_[FFT_LINES * FFT_MAXN], by_[FFT_LINES * FFT_MAXN];
+  __shared__ float2 stw[FFT_MAXN];
+  __shared__ double red[FFT_TB / 64][10];
+  __shared__ int last;
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  if (P.kmode != 1) return;
+  const int nx = P.nx, ny = P.ny, nz = P.nz, nzh = nz / 2 + 1;
+  const int nlines = ny * nzh;
+  const int nblk = (nlines + FFT_LINES - 1) / FFT_LINES;
+  if ((int)blockIdx.x >= nblk) return;
+  load_tw(stw, twid_all + (long)s * 3 * FFT_MAXN, nx);
+  const int l = threadIdx.x / FFT_T, tid = threadIdx.x % FFT_T;
+  const int line = blockIdx.x * FFT_LINES + l;
+  const bool on = line < nlines;
+  const int ky = on ? line / nzh : 0, kz = on ? line % nzh : 0;
+  float2 *xl = bx_ + l * FFT_MAXN;
+  float2 *src = cmesh + P.cmesh0 + (long)ky * nzh + kz;
+  const long xstride = (long)ny * nzh;
+  if (on)
+    for (int t = tid; t < nx; t += FFT_T) xl[t] = src[t * xstride];
+  __syncthreads();
+  float2 *res = lds_fft_lines(bx_, by_, nx, stw, -1);
+  float2 *oth = (res == bx_) ? by_ : bx_;
+  double e = 0, w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (on) {
+    float2 *rl = res + l * FFT_MAXN;
+    const float *g = green + P.green0 + ((long)ky * nzh + kz) * nx;
+    const double mult = (kz == 0 || (nz % 2 == 0 && kz == nz / 2)) ? 1.0 : 2.0;
+    const int jy = ky > ny / 2 ? ky - ny : ky;
+    const double c4 = 0.25 / (P.eta * P.eta);
+    for (int t = tid; t < nx; t += FFT_T) {
+      const float2 v = rl[t];
+      const float gg = g[t];
+      const double ek = 0.5 * mult * (double)gg * ((double)v.x * v.x + (double)v.y * v.y);
+      e += ek;
+      if (want_virial && gg != 0.f) {
+        const int jx = t > nx / 2 ? t - nx : t;
+        double k[3];
+        for (int a = 0; a < 3; a++) k[a] = P.RL[a] * jx + P.RL[3 + a] * jy + P.RL[6 + a] * kz;
+        const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        const double f = 2 * (1 / k2 + c4);
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) w[3 * a + b] += ek * ((a == b ? 1.0 : 0.0) - f * k[a] * k[b]);
+      }
+      rl[t] = make_float2(v.x * gg, v.y * gg);
+    }
+  }
+  __syncthreads();
+  res = lds_fft_lines(res, oth, nx, stw, +1);
+  if (on) {
+    float2 *rl = res + l * FFT_MAXN;
+    for (int t = tid; t < nx; t += FFT_T) src[t * xstride] = rl[t];
+  }
+  // deterministic block partials, summed in block order by the last block
+  for (int o = 32; o >= 1; o >>= 1) {
+    e += __shfl_xor(e, o);
+    for (int c = 0; c < 9; c++) w[c] += __shfl_xor(w[c], o);
+  }
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wid][0] = e;
+    for (int c = 0; c < 9; c++) red[wid][1 + c] = w[c];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int ww = 0; ww < FFT_TB / 64; ww++)
+      for (int c = 0; c < 10; c++) t[c] += red[ww][c];
+    double *pp = part + ((long)s * maxblk + blockIdx.x) * 10;
+    for (int c = 0; c < 10; c++) pp[c] = t[c];
+    __threadfence();
+    last = (atomicAdd(&done[s], 1) == nblk - 1);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    double t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nblk; b++) {
+      const volatile double *pp = part + ((long)s * maxblk + b) * 10;
+      for (int c = 0; c < 10; c++) t[c] += pp[c];
+    }
+    acc[s * NACC + ACC_KSPACE_E] = t[0];
+    for (int c = 0; c < 9; c++) kvir[9 * s + c] = want_virial ? t[1 + c] : 0.0;
+    done[s] = 0;
+  }
+}
+
+__global__ void k_p3m_gather(const SysParams *__restrict__ sp, const int4 *__restrict__ sgrid,
+                             const float4 *__restrict__ sfrac, const float *__restrict__ mesh,
+                             double *__restrict__ kres) {
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  if (P.kmode != 1) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nsite) return;
+  const int si = P.site0 + i;
+  const int4 g = sgrid[si];
+  const float4 f = sfrac[si];
+  const int p = P.order, nx = P.nx, ny = P.ny, nz = P.nz;
+  const int istart = -((p + 1) / 2 - 1);
+  float wx[MAXORDER], wy[MAXORDER], wz[MAXORDER], dx[MAXORDER], dy[MAXORDER], dz[MAXORDER];
+  bspline_w(f.x, p, wx, dx);
+  bspline_w(f.y, p, wy, dy);
+  bspline_w(f.z, p, wz, dz);
+  int iz[MAXORDER];
+  for (int c = 0; c < p; c++) iz[c] = mod_nonneg(g.z + istart + c, nz);
+  float ph = 0.f, e0 = 0.f, e1 = 0.f, e2 = 0.f;
+  for (int a = 0; a < p; a++) {
+    const int ix = mod_nonneg(g.x + istart + a, nx);
+    for (int b = 0; b < p; b++) {
+      const int iy = mod_nonneg(g.y + istart + b, ny);
+      const float *col = mesh + P.mesh0 + ((long)ix * ny + iy) * nz;
+      const float wxy = wx[a] * wy[b];
+      const float dxy = dx[a] * wy[b], wdy = wx[a] * dy[b];
+      float cp = 0.f, cz = 0.f;
+      for (int c = 0; c < p; c++) {
+        const float pm = col[iz[c]];
+        cp += wz[c] * pm;
+        cz += dz[c] * pm;
+      }
+      ph += wxy * cp;
+      e0 += dxy * cp;
+      e1 += wdy * cp;
+      e2 += wxy * cz;
+    }
+  }
+  const double *T = P.T;
+  kres[4 * si] = (double)ph;
+  kres[4 * si + 1] = (double)e0 * T[0] + (double)e1 * T[1] + (double)e2 * T[2];
+  kres[4 * si + 2] = (double)e0 * T[3] + (double)e1 * T[4] + (double)e2 * T[5];
+  kres[4 * si + 3] = (double)e0 * T[6] + (double)e1 * T[7] + (double)e2 * T[8];
+}
+
+int launch_p3m(Engine &e, int want_virial) {
+  int maxn = 0, maxbins = 0, maxz = 0, maxy = 0, maxx = 0;
+  for (auto &p : e.hsp)
+    if (p.kmode == 1) {
+      maxn = std::max(maxn, p.nsite);
+      maxbins = std::max(maxbins, p.nbx * p.nby);
+      const int nzh = p.nz / 2 + 1;
+      maxz = std::max(maxz, (p.nx * p.ny + FFT_LINES - 1) / FFT_LINES);
+      maxy = std::max(maxy, (p.nx * nzh + FFT_LINES - 1) / FFT_LINES);
+      maxx = std::max(maxx, (p.ny * nzh + FFT_LINES - 1) / FFT_LINES);
+    }
+  if (maxn == 0) return 0;
+  const int tb = 256;
+  dim3 gs((maxn + tb - 1) / tb, e.nsys);
+  hipLaunchKernelGGL(k_p3m_grid, gs, dim3(tb), 0, e.stream, e.d_sp, e.d_r, e.d_q, e.nsite,
+                     e.d_sgrid, e.d_sfrac, e.d_bin_count);
+  hipLaunchKernelGGL(k_p3m_binscan, dim3(1, e.nsys), dim3(1024), 0, e.stream, e.d_sp,
+                     e.d_bin_count, e.d_bin_start, e.d_bin_fill);
+  hipLaunchKernelGGL(k_p3m_binscatter, gs, dim3(tb), 0, e.stream, e.d_sp, e.d_sgrid,
+                     e.d_bin_start, e.d_bin_fill, e.d_bin_sites);
+  hipLaunchKernelGGL(k_p3m_spread, dim3(maxbins, e.nsys), dim3(256), 0, e.stream, e.d_sp,
+                     e.d_sgrid, e.d_sfrac, e.d_bin_start, e.d_bin_sites, e.d_mesh);
+  hipLaunchKernelGGL(k_fft_pass, dim3(maxz, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp, e.d_twid,
+                     e.d_mesh, e.d_cmesh, 0);
+  hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp, e.d_twid,
+                     e.d_mesh, e.d_cmesh, 1);
+  hipLaunchKernelGGL(k_fft_xgreen, dim3(maxx, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp,
+                     e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
+                     e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
+  hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp, e.d_twid,
+                     e.d_mesh, e.d_cmesh, 3);
+  hipLaunchKernelGGL(k_fft_pass, dim3(maxz, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp, e.d_twid,
+                     e.d_mesh, e.d_cmesh, 4);
+  hipLaunchKernelGGL(k_p3m_gather, gs, dim3(tb), 0, e.stream, e.d_sp, e.d_sgrid, e.d_sfrac,
+                     e.d_mesh, e.d_kres);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mmamd

@@ -1,0 +1,287 @@
+// rspace.hip -- Ewald real-space erfc-Coulomb + Lennard-Jones pair kernel.
+//
+// Same interaction set and formulas as RSpaceEwald (rspace.c:382-558):
+//   * per neighbour group pair, one minimum-image shift from the first sites
+//     (rspace.c:423-435), FP64, and the same r^2 > rc^2 skip and
+//     (rc-w)^2 < r^2 switching decision (rspace.c:436-445);
+//   * every site pair of the two groups uses that shift (rspace.c:454-478);
+//   * erfc(eta r)/r and -2 d/dr^2 of it from a cubic Chebyshev table in r^2
+//     (cheby.c / rspace.c:19-37; built in FP64 on the host, evaluated in
+//     FP32 here), exact erfc outside the table (rspace.c:116-125);
+//   * Lorentz-Berthelot LJ (rspace.c:258-297) via a per-type-pair table;
+//   * the switching derivative times the sum of unswitched pair energies acts
+//     on the first sites (rspace.c:479-489).
+// MI355X mapping: groups are cell-sorted; 16 lanes serve one i-group and
+// stride over its full neighbour row (both directions, so every force lands
+// on the lane's own i-group and no atomics are needed); the i-group lives
+// in registers; site-pair arithmetic is FP32; each group pair's FP32 partial
+// sums are added into FP64 accumulators; the 16 lanes of an i-group are
+// reduced with cross-lane shuffles at the end.
+#include "engine.hpp"
+
+namespace mmamd {
+
+// ---- per-step pack: cell-sorted FP64 first sites + FP32 site offsets ----
+__global__ void k_pack(const SysParams *__restrict__ sp, const int *__restrict__ order,
+                       const int *__restrict__ gfirst, const int *__restrict__ gns,
+                       const int *__restrict__ ginfo, const double *__restrict__ r,
+                       const double *__restrict__ q, int N, double4 *__restrict__ gpos,
+                       float4 *__restrict__ spk, int2 *__restrict__ gmeta) {
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.ngroup) return;
+  const int gs = P.group0 + i;
+  const int g = order[gs];
+  const int s0 = gfirst[g], ns = gns[g];
+  const double x0 = r[s0], y0 = r[N + s0], z0 = r[2 * N + s0];
+  gpos[gs] = make_double4(x0, y0, z0, 0.0);
+#pragma unroll
+  for (int k = 0; k < MAXS; k++) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (k < ns) {
+      const int sk = s0 + k;
+      v = make_float4((float)(r[sk] - x0), (float)(r[N + sk] - y0), (float)(r[2 * N + sk] - z0),
+                      (float)q[sk]);
+    }
+    spk[(long)gs * MAXS + k] = v;
+  }
+  gmeta[gs] = make_int2(ns, ginfo[g]);
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 16); }
+
+constexpr int NSI = 3;       // sites per group handled by the pair kernel
+constexpr int LANES = 16;    // lanes per i-group
+constexpr int PAIR_TB = 256;
+constexpr int GPB = PAIR_TB / LANES;   // i-groups per block
+constexpr int MAXTAB = 1536;
+constexpr int MAXLJ = 64;
+
+template <bool FIELD, bool VIRIAL>
+__global__ void __launch_bounds__(PAIR_TB) k_pair(
+    const SysParams *__restrict__ sp, const double4 *__restrict__ gpos,
+    const float4 *__restrict__ spk, const int2 *__restrict__ gmeta,
+    const int *__restrict__ nbr, const int *__restrict__ nnb, int nbcap,
+    const float4 *__restrict__ tab, const float2 *__restrict__ ljt,
+    double *__restrict__ pf, double *__restrict__ plj, double *__restrict__ gacc) {
+  __shared__ float4 stab[MAXTAB];
+  __shared__ float2 slj[MAXLJ];
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  if ((int)(blockIdx.x * GPB) >= P.ngroup) return;
+  const int ntab = P.ntabA + P.ntabB;
+  for (int t = threadIdx.x; t < ntab; t += PAIR_TB) stab[t] = tab[P.tab0 + t];
+  for (int t = threadIdx.x; t < P.ntype * P.ntype; t += PAIR_TB) slj[t] = ljt[P.lj0 + t];
+  __syncthreads();
+
+  const int lane = threadIdx.x & (LANES - 1);
+  const int i = blockIdx.x * GPB + (threadIdx.x / LANES);
+  const bool active = i < P.ngroup;
+  const int gs = P.group0 + (active ? i : 0);
+
+  const double4 pi = gpos[gs];
+  const int2 mi = gmeta[gs];
+  const int nsi = mi.x;
+  float4 oi[NSI];
+  int mki[NSI], tyi[NSI];
+#pragma unroll
+  for (int k = 0; k < NSI; k++) {
+    oi[k] = spk[(long)gs * MAXS + k];
+    const int b = (mi.y >> (8 * k)) & 0xff;
+    mki[k] = k < nsi ? (b & 3) : 0;
+    tyi[k] = b >> 2;
+  }
+  // accumulators
+  double ax[NSI], ay[NSI], az[NSI], ap[NSI];
+  double lx[NSI], ly[NSI], lz[NSI];
+  double ulj = 0;
+  double w[9];
+#pragma unroll
+  for (int k = 0; k < NSI; k++) { ax[k] = ay[k] = az[k] = ap[k] = 0; lx[k] = ly[k] = lz[k] = 0; }
+#pragma unroll
+  for (int c = 0; c < 9; c++) w[c] = 0;
+
+  const double L = P.L, invL = P.invL, rhi2 = P.rhi2, rlo2 = P.rlo2;
+  const float tAmin = P.tabAmin, tMid = P.tabMid, tAinv = P.tabAinv, tBinv = P.tabBinv,
+              tBmax = P.tabBmax;
+  const int ntA = P.ntabA, nty = P.ntype;
+  const float eta = (float)P.eta;
+  const int cnt = active ? nnb[gs] : 0;
+  const int *row = nbr + (long)gs * nbcap;
+
+  for (int t = lane; t < cnt; t += LANES) {
+    const int hs = row[t];
+    const double4 pj = gpos[hs];
+    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    dx -= L * floor(dx * invL + 0.5);
+    dy -= L * floor(dy * invL + 0.5);
+    dz -= L * floor(dz * invL + 0.5);
+    const double r2g = dx * dx + dy * dy + dz * dz;
+    if (r2g > rhi2) continue;
+    double sm = 1.0, dsm = 0.0;
+    if (r2g > rlo2) {                       // rspace.c:52-58
+      const double a = P.inv_w2;
+      const double z = (r2g - rlo2) * a;
+      sm = ((-6 * z + 15) * z - 10) * z * z * z + 1;
+      dsm = ((60 * z - 120) * z + 60) * z * z * a;
+    }
+    const float smf = (float)sm;
+    const int2 mj = gmeta[hs];
+    const int nsj = mj.x;
+    const float Dx = (float)dx, Dy = (float)dy, Dz = (float)dz;
+    float fx[NSI], fy[NSI], fz[NSI], ph[NSI], gx[NSI], gy[NSI], gz[NSI];
+#pragma unroll
+    for (int k = 0; k < NSI; k++) { fx[k] = fy[k] = fz[k] = ph[k] = 0.f; gx[k] = gy[k] = gz[k] = 0.f; }
+    float tmps = 0.f, uu = 0.f;
+#pragma unroll
+    for (int l = 0; l < NSI; l++) {
+      if (l >= nsj) break;
+      const float4 oj = spk[(long)hs * MAXS + l];
+      const int bj = (mj.y >> (8 * l)) & 0xff;
+      const int mkj = bj & 3, tyj = bj >> 2;
+      const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
+      const float qj = oj.w;
+#pragma unroll
+      for (int k = 0; k < NSI; k++) {
+        const int m = mki[k] & mkj;
+        if (!m) continue;
+        const float rx = px + oi[k].x, ry = py + oi[k].y, rz = pz + oi[k].z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (m & SITE_HAS_Q) {
+          float b0, b1;
+          if (r2 >= tAmin && r2 < tBmax) {
+            const bool hi = r2 >= tMid;
+            const float inv = hi ? tBinv : tAinv;
+            const float xx = (r2 - (hi ? tMid : tAmin)) * inv;
+            const int ii = (int)xx;
+            const float fr = xx - (float)ii;
+            const float4 c = stab[(hi ? ntA : 0) + ii];
+            b0 = ((c.w * fr + c.z) * fr + c.y) * fr + c.x;
+            b1 = -2.f * inv * ((3.f * c.w * fr + 2.f * c.z) * fr + c.y);
+          } else {                            // rspace.c:120-125, exact
+            const double rm = sqrt((double)r2), er = (double)eta * rm;
+            const double e0 = erfc(er) / rm;
+            b0 = (float)e0;
+            b1 = (float)((e0 + P.eta * 1.1283791670955126 * exp(-er * er)) / (double)r2);
+          }
+          const float qiqj = oi[k].w * qj;
+          tmps += qiqj * b0;
+          const float b0s = b0 * smf, b1s = b1 * smf;
+          ph[k] += qj * b0s;
+          const float e = qj * b1s;
+          fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
+        }
+        if (m & SITE_HAS_LJ) {
+          const float2 c = slj[tyi[k] * nty + tyj];
+          const float ir2 = 1.f / r2;
+          const float ir6 = ir2 * ir2 * ir2;
+          const float bi6 = c.x * ir6, ai12 = c.y * ir6 * ir6;
+          const float u = ai12 - bi6;
+          const float du = (12.f * ai12 - 6.f * bi6) * ir2 * smf;
+          tmps += u;
+          uu += u * smf;
+          gx[k] += du * rx; gy[k] += du * ry; gz[k] += du * rz;
+        }
+      }
+    }
+    // smoothing force on the first site (rspace.c:479-489)
+    const double fs = dsm * (double)tmps;
+    const double sx = fs * dx, sy = fs * dy, sz = fs * dz;
+    double Fx = sx, Fy = sy, Fz = sz;
+#pragma unroll
+    for (int k = 0; k < NSI; k++) {
+      ap[k] += (double)ph[k];
+      if (FIELD) {
+        ax[k] += (double)fx[k]; ay[k] += (double)fy[k]; az[k] += (double)fz[k];
+        lx[k] += (double)gx[k]; ly[k] += (double)gy[k]; lz[k] += (double)gz[k];
+      } else {
+        const float qk = oi[k].w;
+        const double tx = (double)(qk * fx[k] + gx[k]), ty = (double)(qk * fy[k] + gy[k]),
+                     tz = (double)(qk * fz[k] + gz[k]);
+        ax[k] += tx; ay[k] += ty; az[k] += tz;
+        if (VIRIAL) { Fx += tx; Fy += ty; Fz += tz; }
+      }
+    }
+    if (FIELD) { lx[0] += sx; ly[0] += sy; lz[0] += sz; }
+    else { ax[0] += sx; ay[0] += sy; az[0] += sz; }
+    ulj += (double)uu;
+    if (VIRIAL) {   // 1/2 D (x) F_gh per directed pair
+      w[0] += 0.5 * dx * Fx; w[1] += 0.5 * dx * Fy; w[2] += 0.5 * dx * Fz;
+      w[3] += 0.5 * dy * Fx; w[4] += 0.5 * dy * Fy; w[5] += 0.5 * dy * Fz;
+      w[6] += 0.5 * dz * Fx; w[7] += 0.5 * dz * Fy; w[8] += 0.5 * dz * Fz;
+    }
+  }
+  // reduce the 16 lanes of each i-group
+#pragma unroll
+  for (int m = LANES / 2; m >= 1; m >>= 1) {
+#pragma unroll
+    for (int k = 0; k < NSI; k++) {
+      ax[k] += shfl_xor_d(ax[k], m); ay[k] += shfl_xor_d(ay[k], m);
+      az[k] += shfl_xor_d(az[k], m); ap[k] += shfl_xor_d(ap[k], m);
+      if (FIELD) {
+        lx[k] += shfl_xor_d(lx[k], m); ly[k] += shfl_xor_d(ly[k], m);
+        lz[k] += shfl_xor_d(lz[k], m);
+      }
+    }
+    ulj += shfl_xor_d(ulj, m);
+    if (VIRIAL) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) w[c] += shfl_xor_d(w[c], m);
+    }
+  }
+  if (!active || lane != 0) return;
+#pragma unroll
+  for (int k = 0; k < NSI; k++) {
+    double *o = pf + ((long)gs * MAXS + k) * 4;
+    o[0] = ax[k]; o[1] = ay[k]; o[2] = az[k]; o[3] = ap[k];
+    if (FIELD) {
+      double *l = plj + ((long)gs * MAXS + k) * 3;
+      l[0] = lx[k]; l[1] = ly[k]; l[2] = lz[k];
+    }
+  }
+  double *ga = gacc + (long)gs * 10;
+  ga[0] = 0.5 * ulj;   // each unordered pair is visited from both sides
+  if (VIRIAL) {
+    // per-site offset term: sum_k o_k (x) F_k (o_0 = 0)
+#pragma unroll
+    for (int k = 1; k < NSI; k++) {
+      if (k >= nsi) break;
+      const double ox = oi[k].x, oy = oi[k].y, oz = oi[k].z;
+      w[0] += ox * ax[k]; w[1] += ox * ay[k]; w[2] += ox * az[k];
+      w[3] += oy * ax[k]; w[4] += oy * ay[k]; w[5] += oy * az[k];
+      w[6] += oz * ax[k]; w[7] += oz * ay[k]; w[8] += oz * az[k];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; c++) ga[1 + c] = w[c];
+  }
+}
+
+int launch_pack(Engine &e) {
+  const int tb = 256;
+  hipLaunchKernelGGL(k_pack, dim3((e.maxg + tb - 1) / tb, e.nsys), dim3(tb), 0, e.stream, e.d_sp,
+                     e.d_order, e.d_gfirst, e.d_gns, e.d_ginfo, e.d_r, e.d_q, e.nsite, e.d_gpos,
+                     e.d_spk, e.d_gmeta);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_pair(Engine &e, int field_mode, int want_virial) {
+  dim3 grid((e.maxg + GPB - 1) / GPB, e.nsys), tb(PAIR_TB);
+  if (field_mode)
+    hipLaunchKernelGGL((k_pair<true, false>), grid, tb, 0, e.stream, e.d_sp, e.d_gpos, e.d_spk,
+                       e.d_gmeta, e.d_nbr, e.d_nnb, e.nbcap, e.d_tab, e.d_ljt, e.d_pf, e.d_plj,
+                       e.d_gacc);
+  else if (want_virial)
+    hipLaunchKernelGGL((k_pair<false, true>), grid, tb, 0, e.stream, e.d_sp, e.d_gpos, e.d_spk,
+                       e.d_gmeta, e.d_nbr, e.d_nnb, e.nbcap, e.d_tab, e.d_ljt, e.d_pf, e.d_plj,
+                       e.d_gacc);
+  else
+    hipLaunchKernelGGL((k_pair<false, false>), grid, tb, 0, e.stream, e.d_sp, e.d_gpos, e.d_spk,
+                       e.d_gmeta, e.d_nbr, e.d_nnb, e.nbcap, e.d_tab, e.d_ljt, e.d_pf, e.d_plj,
+                       e.d_gacc);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mmamd

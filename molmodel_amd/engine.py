@@ -1,0 +1,212 @@
+"""Python access to libmolmodel_amd.so through its C ABI (include/molmodel_amd.h).
+
+This is plumbing for tests, benchmarks and scripting; all computation runs in
+the HIP library.  Importing this module never falls back to anything else:
+if the library is missing or no GPU is present, creating an Engine raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmolmodel_amd.so")
+
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_L = C.c_long
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [("nsite", _I), ("ngroup", _I), ("group", _P), ("q", _P), ("sigma", _P),
+                ("epsilon", _P), ("site_mask", _P), ("mass", _P), ("nexcl", _I), ("excl", _P),
+                ("boxl", _D), ("rc", _D), ("smoothing_width", _D), ("skin", _D), ("eta", _D),
+                ("kmode", _I), ("kc", _D), ("mesh", _I * 3), ("order", _I), ("nalias", _I),
+                ("ik", _I), ("include_lj_correction", _I), ("geometric_sigma", _I)]
+
+
+NE = 8
+E_ELEC, E_LJ, E_TAIL, E_TOTAL, E_KINETIC, E_KSPACE = range(6)
+VIRIAL = 1
+
+_lib = None
+
+
+def lib():
+    """Load the built library (fails loudly if it is absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run molmodel_amd.build.build() first")
+        L = C.CDLL(LIB_PATH)
+        L.mm_last_error.restype = C.c_char_p
+        L.mm_engine_create.argtypes = [_I, _P, C.POINTER(_P)]
+        L.mm_engine_destroy.argtypes = [_P]
+        for name in ("mm_set_positions", "mm_get_positions", "mm_set_velocities",
+                     "mm_get_velocities", "mm_get_forces", "mm_get_potential",
+                     "mm_get_energies", "mm_get_virial"):
+            getattr(L, name).argtypes = [_P, _P]
+        L.mm_compute.argtypes = [_P, _I]
+        L.mm_errors.argtypes = [_P, _P]
+        L.mm_rspace_field.argtypes = [_P, _P, _P, _P, _P]
+        L.mm_kspace_field.argtypes = [_P, _P, _P]
+        L.mm_nl_pairs.restype = _L
+        L.mm_nl_pairs.argtypes = [_P, _I, _P, _L]
+        L.mm_md_setup.argtypes = [_P, _D, _D, _D, _D]
+        L.mm_md_run.argtypes = [_P, _I, _I]
+        L.mm_stream.restype = _P
+        L.mm_stream.argtypes = [_P]
+        L.mm_sync.argtypes = [_P]
+        L.mm_engine_sizes.argtypes = [_P, _P, _P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+class MMError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc < 0:
+        raise MMError(f"molmodel_amd error {rc}: {lib().mm_last_error().decode()}")
+    return rc
+
+
+def system_desc(system, params):
+    """Build the C descriptor for a waterbox.System and method parameters;
+    returns (desc, keepalive arrays)."""
+    keep = dict(
+        group=np.ascontiguousarray(system.group, dtype=np.int32).reshape(-1),
+        q=np.ascontiguousarray(system.q, dtype=np.float64),
+        sig=np.ascontiguousarray(system.sig, dtype=np.float64),
+        eps=np.ascontiguousarray(system.eps, dtype=np.float64),
+        mask=np.ascontiguousarray(system.site_mask, dtype=np.int32),
+        mass=np.ascontiguousarray(system.mass, dtype=np.float64),
+        excl=np.ascontiguousarray(system.excl, dtype=np.int32).reshape(-1),
+    )
+    mesh = params.get("mesh", (0, 0, 0))
+    d = SystemDesc(system.nsite, system.ngroup, _ptr(keep["group"]), _ptr(keep["q"]),
+                   _ptr(keep["sig"]), _ptr(keep["eps"]), _ptr(keep["mask"]), _ptr(keep["mass"]),
+                   len(keep["excl"]) // 2, _ptr(keep["excl"]), system.boxl, params["rc"],
+                   params["smoothing_width"], params["skin"], params["eta"], params["kmode"],
+                   params.get("kc", 0.0), (_I * 3)(*mesh), params.get("order", 5),
+                   params.get("nalias", 2), params.get("ik", 0),
+                   params.get("include_lj_correction", 1), params.get("geometric_sigma", 0))
+    return d, keep
+
+
+class Engine:
+    """A batch of systems resident on the GPU."""
+
+    def __init__(self, systems, params):
+        if not isinstance(systems, (list, tuple)):
+            systems, params = [systems], [params]
+        elif not isinstance(params, (list, tuple)):
+            params = [params] * len(systems)
+        self.systems = list(systems)
+        descs, self._keep = [], []
+        for s, p in zip(systems, params):
+            d, k = system_desc(s, p)
+            descs.append(d)
+            self._keep.append(k)
+        arr = (SystemDesc * len(descs))(*descs)
+        h = _P()
+        _check(lib().mm_engine_create(len(descs), C.cast(arr, _P), C.byref(h)))
+        self.h = h
+        self.nsys = len(descs)
+        self.nsite = sum(s.nsite for s in systems)
+        self.offsets = np.cumsum([0] + [s.nsite for s in systems])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mm_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state ---------------------------------------------------------------
+    def set_positions(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1)
+        assert r.size == 3 * self.nsite
+        _check(lib().mm_set_positions(self.h, _ptr(r)))
+
+    def positions(self):
+        r = np.zeros((self.nsite, 3))
+        _check(lib().mm_get_positions(self.h, _ptr(r)))
+        return r
+
+    def set_velocities(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        _check(lib().mm_set_velocities(self.h, _ptr(v)))
+
+    def velocities(self):
+        v = np.zeros((self.nsite, 3))
+        _check(lib().mm_get_velocities(self.h, _ptr(v)))
+        return v
+
+    # evaluation ------------------------------------------------------------
+    def compute(self, virial=False):
+        _check(lib().mm_compute(self.h, VIRIAL if virial else 0))
+
+    def forces(self):
+        f = np.zeros((self.nsite, 3))
+        _check(lib().mm_get_forces(self.h, _ptr(f)))
+        return f
+
+    def potential(self):
+        p = np.zeros(self.nsite)
+        _check(lib().mm_get_potential(self.h, _ptr(p)))
+        return p
+
+    def energies(self):
+        e = np.zeros((self.nsys, NE))
+        _check(lib().mm_get_energies(self.h, _ptr(e)))
+        return e
+
+    def virial(self):
+        w = np.zeros((self.nsys, 9))
+        _check(lib().mm_get_virial(self.h, _ptr(w)))
+        return w.reshape(self.nsys, 3, 3)
+
+    def errors(self):
+        f = np.zeros(1, dtype=np.int32)
+        _check(lib().mm_errors(self.h, _ptr(f)))
+        return int(f[0])
+
+    def rspace_field(self):
+        phi = np.zeros(self.nsite); ev = np.zeros((self.nsite, 3)); ljf = np.zeros((self.nsite, 3))
+        lje = np.zeros(self.nsys)
+        _check(lib().mm_rspace_field(self.h, _ptr(phi), _ptr(ev), _ptr(ljf), _ptr(lje)))
+        return phi, ev, ljf, lje
+
+    def kspace_field(self):
+        phi = np.zeros(self.nsite); ev = np.zeros((self.nsite, 3))
+        _check(lib().mm_kspace_field(self.h, _ptr(phi), _ptr(ev)))
+        return phi, ev
+
+    def nl_pairs(self, sys=0):
+        n = _check(lib().mm_nl_pairs(self.h, sys, None, 0))
+        out = np.zeros((max(n, 1), 2), dtype=np.int32)
+        _check(lib().mm_nl_pairs(self.h, sys, _ptr(out), n))
+        return out[:n]
+
+    # dynamics --------------------------------------------------------------
+    def md_setup(self, dt=0.001, bond=1.0, angle=109.47, tol=1e-8):
+        _check(lib().mm_md_setup(self.h, dt, bond, angle, tol))
+
+    def md_run(self, nsteps, graph=True):
+        _check(lib().mm_md_run(self.h, int(nsteps), 1 if graph else 0))
+
+    def sync(self):
+        _check(lib().mm_sync(self.h))
