@@ -132,6 +132,18 @@ long long mm_nl_builds(mm_engine *e);
 int mm_md_setup(mm_engine *e, double dt, double bond, double angle_deg, double tol);
 int mm_md_run(mm_engine *e, int nsteps, int use_graph);
 
+/* interaction counts of the current neighbour list, 3 per system:
+ * group pairs with first-site r^2 <= rc^2 (unordered), charged site pairs
+ * and LJ site pairs among them */
+int mm_count_pairs(mm_engine *e, long long *counts);
+
+/* average device time of one launch (HIP events on the engine stream) of
+ * the real-space pair kernel (MM_KERNEL_PAIR) or of the whole P3M sequence
+ * (MM_KERNEL_P3M), over reps back-to-back launches at the current state */
+#define MM_KERNEL_PAIR 0
+#define MM_KERNEL_P3M 1
+int mm_time_kernel(mm_engine *e, int which, int reps, double *ms_per_launch);
+
 /* stream the engine runs on (hipStream_t) and device pointers of the FP64
  * SoA state (x[0..N), y[N..2N), z[2N..3N)) for device-resident callers */
 void *mm_stream(mm_engine *e);

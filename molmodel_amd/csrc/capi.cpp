@@ -194,7 +194,14 @@ static int grow_nbr(Engine &e) {
 
 static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
   Engine &e = m->e;
-  if (e.first_eval) {
+  if (e.no_rspace) {
+    TRY(launch_kspace(e, want_virial));
+    TRY(launch_p3m(e, want_virial));
+    return 0;
+  }
+  if (e.external_nl) {
+    e.first_eval = 0;
+  } else if (e.first_eval) {
     TRY(launch_nl_build(e, 1));
     e.first_eval = 0;
   } else {
@@ -432,5 +439,39 @@ int mm_sync(mm_engine *m) {
   return MM_OK;
 }
 long long mm_nl_builds(mm_engine *m) { return m ? 0 : -1; }
+
+int mm_count_pairs(mm_engine *m, long long *counts) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  if (e.first_eval || m->pos_dirty) TRY(compute_checked(m, 0, 0));
+  TRY(scratch(e, 3 * (size_t)e.nsys));
+  unsigned long long *d = (unsigned long long *)e.d_scratch;
+  MM_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long) * 3 * e.nsys, e.stream));
+  TRY(launch_count_pairs(e, d));
+  MM_HIP(hipMemcpyAsync(counts, d, sizeof(long long) * 3 * e.nsys, hipMemcpyDeviceToHost, e.stream));
+  MM_HIP(hipStreamSynchronize(e.stream));
+  return MM_OK;
+}
+
+int mm_time_kernel(mm_engine *m, int which, int reps, double *ms_per_launch) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  if (reps <= 0 || !ms_per_launch) { set_last_error("bad arguments"); return MM_ERR_ARG; }
+  if (e.first_eval || m->pos_dirty) TRY(compute_checked(m, 0, 0));
+  hipEvent_t a, b;
+  MM_HIP(hipEventCreate(&a));
+  MM_HIP(hipEventCreate(&b));
+  TRY(which == MM_KERNEL_PAIR ? launch_pair(e, 0, 0) : launch_p3m(e, 0));   // warm
+  MM_HIP(hipEventRecord(a, e.stream));
+  for (int i = 0; i < reps; i++) TRY(which == MM_KERNEL_PAIR ? launch_pair(e, 0, 0) : launch_p3m(e, 0));
+  MM_HIP(hipEventRecord(b, e.stream));
+  MM_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  MM_HIP(hipEventElapsedTime(&ms, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  *ms_per_launch = ms / reps;
+  return MM_OK;
+}
 
 }  // extern "C"

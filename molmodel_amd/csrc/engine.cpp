@@ -82,14 +82,10 @@ static std::vector<int4> nl_stencil(int ncell, double boxl, double rc, double sk
 }
 
 // ---- erfc(eta r)/r table (cheby.c:7-33 fit; rspace.c:25-37 ranges) ----
-static void cheb_fit(double a, double b, double eta, double *c4) {
+void cheb_fit(double a, double b, double (*f)(double), double *c4) {
   double g[50], c[4];
   const double fac = 2.0 / 50, bma = 0.5 * (b - a), bpa = 0.5 * (b + a);
-  for (int k = 0; k < 50; k++) {
-    const double r2 = cos(M_PI * (k + 0.5) / 50) * bma + bpa;
-    const double r = sqrt(r2);
-    g[k] = erfc(eta * r) / r;
-  }
+  for (int k = 0; k < 50; k++) g[k] = f(cos(M_PI * (k + 0.5) / 50) * bma + bpa);
   for (int j = 0; j < 4; j++) {
     double s = 0;
     for (int k = 0; k < 50; k++) s += g[k] * cos(M_PI * j * (k + 0.5) / 50);
@@ -101,7 +97,14 @@ static void cheb_fit(double a, double b, double eta, double *c4) {
   c4[3] = 32 * c[3];
 }
 
+static thread_local double tab_eta;
+static double erfc_over_r(double r2) {
+  const double r = sqrt(r2);
+  return erfc(tab_eta * r) / r;
+}
+
 static void build_table(SysParams &P, double rc, double eta, std::vector<float4> &tab) {
+  tab_eta = eta;
   const double r2max = 1.2 * sq(rc);
   const double r2min = r2max > 4 ? 4 : 0.9 * r2max;
   // the reference's interval width, widened only if the extended range
@@ -124,17 +127,17 @@ static void build_table(SysParams &P, double rc, double eta, std::vector<float4>
   P.tabBmax = (float)(r2min + nB * dxB) * (1.0f - 4e-7f);
   double c[4];
   for (int i = 0; i < nA; i++) {
-    cheb_fit(amin + i * dxA, amin + (i + 1) * dxA, eta, c);
+    cheb_fit(amin + i * dxA, amin + (i + 1) * dxA, erfc_over_r, c);
     tab.push_back(make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[3]));
   }
   for (int i = 0; i < nB; i++) {
-    cheb_fit(i * dxB + r2min, (i + 1) * dxB + r2min, eta, c);
+    cheb_fit(i * dxB + r2min, (i + 1) * dxB + r2min, erfc_over_r, c);
     tab.push_back(make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[3]));
   }
 }
 
 // ---- tensors in tensor_t layout (cart.h:73-92) ----
-static void tensor_inverse(double *a, const double *b) {
+void tensor_inverse(double *a, const double *b) {
   const double xx = b[0], yx = b[1], zx = b[2], xy = b[3], yy = b[4], zy = b[5], xz = b[6],
                yz = b[7], zz = b[8];
   const double det = -xz * yy * zx + xy * yz * zx + xz * yx * zy - xx * yz * zy - xy * yx * zz + xx * yy * zz;
@@ -156,8 +159,8 @@ static double sincpi(double x) {   // pppm.c:56-59
 }
 
 // optimal influence function at signed index (pppm.c:331-388)
-static double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl,
-                       double inv_4eta2, double inv_v, int nalias, int order) {
+double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl,
+                double inv_4eta2, double inv_v, int nalias, int order, double *chi_acc) {
   if (jx == 0 && jy == 0 && jz == 0) return 0;
   double ux[16], uy[16], uz[16];
   for (int qq = -nalias; qq <= nalias; qq++) {
@@ -165,7 +168,7 @@ static double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const dou
     uy[qq + nalias] = pow(sincpi((double)jy / ny + qq), 2 * order);
     uz[qq + nalias] = pow(sincpi((double)jz / nz + qq), 2 * order);
   }
-  double numer = 0, d1 = 0, d2 = 0;
+  double numer = 0, d1 = 0, d2 = 0, chi = 0;
   for (int px = -nalias; px <= nalias; px++) {
     const double pxx = jx + px * nx;
     for (int py = -nalias; py <= nalias; py++) {
@@ -182,9 +185,11 @@ static double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const dou
         numer += gg * kp2 * u2;
         d1 += u2;
         d2 += u2 * kp2;
+        chi += gg * gg * kp2;
       }
     }
   }
+  if (chi_acc) *chi_acc += chi - sq(numer) / (d1 * d2);
   return inv_v * numer / (d1 * d2);
 }
 
@@ -385,7 +390,13 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
     build_table(P, S.rc, S.eta, tab);
     MM_REQ(P.ntabA + P.ntabB <= 1536, "erfc table too large for the box/cutoff");
     // lattice tensors
-    const double latv[9] = {S.boxl, 0, 0, 0, S.boxl, 0, 0, 0, S.boxl};
+    double latv[9] = {S.boxl, 0, 0, 0, S.boxl, 0, 0, 0, S.boxl};
+    if (S.has_latv) {
+      for (int a = 0; a < 9; a++) latv[a] = S.latv[a];
+      P.volume = fabs(-latv[6] * latv[4] * latv[2] + latv[3] * latv[7] * latv[2] +
+                      latv[6] * latv[1] * latv[5] - latv[0] * latv[7] * latv[5] -
+                      latv[3] * latv[1] * latv[8] + latv[0] * latv[4] * latv[8]);
+    }
     double inv[9];
     tensor_inverse(inv, latv);
     for (int a = 0; a < 3; a++)
@@ -436,7 +447,8 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
           const int jy = iy > S.ny / 2 ? iy - S.ny : iy;
           for (int iz = 0; iz < nzh; iz++)
             gh[((size_t)iy * nzh + iz) * S.nx + ix] =
-                (float)green_at(jx, jy, iz, S.nx, S.ny, S.nz, P.RL, inv_4eta2, inv_v, S.nalias, S.order);
+                (float)green_at(jx, jy, iz, S.nx, S.ny, S.nz, P.RL, inv_4eta2, inv_v, S.nalias, S.order,
+                                nullptr);
         }
       }
       green.insert(green.end(), gh.begin(), gh.end());
@@ -453,7 +465,7 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
       P.bin0 = bin0;
       bin0 += P.nbx * P.nby;
     } else {
-      MM_REQ(false, "kmode must be 0 (Ewald) or 1 (P3M)");
+      MM_REQ(S.kmode == 2, "kmode must be 0 (Ewald), 1 (P3M) or 2 (none)");
     }
     P.dt = 0.001;
     e.hsp.push_back(P);

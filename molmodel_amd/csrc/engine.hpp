@@ -44,6 +44,8 @@ struct SystemInput {       // mirrors mm_system_desc of include/molmodel_amd.h
   int nx, ny, nz, order, nalias, ik;
   int include_lj_correction;
   int geometric_sigma;
+  int has_latv = 0;          // P3M / k-space lattice other than the cubic box
+  double latv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 struct Engine {
@@ -112,6 +114,8 @@ struct Engine {
   double bond = 1.0, angle = 109.47, tol = 1e-8;
   int md_ready = 0;
   int first_eval = 1;
+  int no_rspace = 0;         // reciprocal-space only (kspace_* / p3m_* drop-ins)
+  int external_nl = 0;       // neighbour rows supplied by the caller (RSpaceEwald drop-in)
   int md_phase = 0;          // 0: r, v, F at the same step; 1: drifted half step ahead
   double *d_phi = nullptr;   // per-site total potential of the last evaluation
   double *d_scratch = nullptr; size_t scratch_n = 0;
@@ -122,6 +126,11 @@ struct Engine {
 };
 
 int engine_create(Engine &e, const std::vector<SystemInput> &in);
+// host helpers shared with the reference-named entry points
+void tensor_inverse(double *a, const double *b);
+double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl,
+                double inv_4eta2, double inv_v, int nalias, int order, double *chi_acc);
+void cheb_fit(double a, double b, double (*f)(double), double *c4);
 void engine_destroy(Engine &e);
 
 // launchers (each returns 0 or a negative status)
@@ -134,6 +143,7 @@ int launch_finish(Engine &e, int mode);
 int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out);
 int launch_field_out(Engine &e, double *out, double *ulj);
 int launch_skin(Engine &e);
+int launch_count_pairs(Engine &e, unsigned long long *d_out);
 int launch_set_rebuild(Engine &e);
 int launch_constrain_initial(Engine &e);
 int launch_start_md(Engine &e);

@@ -257,6 +257,52 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
   }
 }
 
+// interaction counts of the current pair list, per system: group pairs with
+// first-site r^2 <= rc^2 (each unordered pair once), charged site pairs and
+// LJ site pairs among them
+__global__ void k_count_pairs(const SysParams *__restrict__ sp, const double4 *__restrict__ gpos,
+                              const int2 *__restrict__ gmeta, const int *__restrict__ nbr,
+                              const int *__restrict__ nnb, int nbcap,
+                              unsigned long long *__restrict__ out) {
+  const int s = blockIdx.y;
+  const SysParams &P = sp[s];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.ngroup) return;
+  const int gs = P.group0 + i;
+  const double4 pi = gpos[gs];
+  const int2 mi = gmeta[gs];
+  unsigned long long ng = 0, nq = 0, nl = 0;
+  for (int t = 0; t < nnb[gs]; t++) {
+    const int hs = nbr[(long)gs * nbcap + t];
+    if (hs <= gs) continue;
+    const double4 pj = gpos[hs];
+    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    dx -= P.L * floor(dx * P.invL + 0.5);
+    dy -= P.L * floor(dy * P.invL + 0.5);
+    dz -= P.L * floor(dz * P.invL + 0.5);
+    if (dx * dx + dy * dy + dz * dz > P.rhi2) continue;
+    ng++;
+    const int2 mj = gmeta[hs];
+    for (int k = 0; k < mi.x; k++)
+      for (int l = 0; l < mj.x; l++) {
+        const int m = ((mi.y >> (8 * k)) & 3) & ((mj.y >> (8 * l)) & 3);
+        nq += (m & SITE_HAS_Q) ? 1 : 0;
+        nl += (m & SITE_HAS_LJ) ? 1 : 0;
+      }
+  }
+  atomicAdd(&out[3 * s], ng);
+  atomicAdd(&out[3 * s + 1], nq);
+  atomicAdd(&out[3 * s + 2], nl);
+}
+
+int launch_count_pairs(Engine &e, unsigned long long *d_out) {
+  const int tb = 256;
+  hipLaunchKernelGGL(k_count_pairs, dim3((e.maxg + tb - 1) / tb, e.nsys), dim3(tb), 0, e.stream,
+                     e.d_sp, e.d_gpos, e.d_gmeta, e.d_nbr, e.d_nnb, e.nbcap, d_out);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_pack(Engine &e) {
   const int tb = 256;
   hipLaunchKernelGGL(k_pack, dim3((e.maxg + tb - 1) / tb, e.nsys), dim3(tb), 0, e.stream, e.d_sp,

@@ -1158,38 +1158,55 @@ double orc_e_to_charge_unit(double e) { return e * sqrt(0.5291772108 * 4.3597441
 double orc_g_mol_to_mass_unit(double m) { return m / (100 * 4.184); }
 double orc_K_to_kcal_mol(double t) { return t * (1.3806505 * 6.0221415 * 1e-3 / 4.184); }
 
-/* eleclj.cpp:785-807 + 912-959 for a cubic box, fresh NL */
-int orc_evaluate(const orc_params_t *p, int nsite, int ngroup, const int *group,
-                 const double *r, const double *q, const double *sig, const double *eps,
-                 const int *site_mask, int nexcl, const int *excl,
-                 double *f, double *energies, double *virial)
+/* Rows of a neighbor list visited as RSpaceEwald visits them: rows in
+ * ascending order, each row in append order (rspace.c:413-418). */
+static int *nl_rows(int ngroup, long np, const int *pairs)
 {
-  const double L = p->boxl, V = L * L * L;
-  double *gr = malloc(sizeof(double) * 3 * (size_t)ngroup);
-  int g, i;
-  for (g = 0; g < ngroup; g++) {
-    gr[3 * g] = r[3 * group[2 * g]];
-    gr[3 * g + 1] = r[3 * group[2 * g] + 1];
-    gr[3 * g + 2] = r[3 * group[2 * g] + 2];
-  }
-  long np = orc_nl_build(ngroup, gr, L, p->rc, p->skin, NULL, 0);
-  if (np < 0) { free(gr); return -1; }
-  int *pairs = malloc(sizeof(int) * 2 * (size_t)(np > 0 ? np : 1));
-  orc_nl_build(ngroup, gr, L, p->rc, p->skin, pairs, np);
-  /* visit rows in ascending order, each row in append order (RSpaceEwald
-   * walks neighbors[ig] for ig = 0..ngroup-1) */
   int *cnt = calloc((size_t)ngroup + 1, sizeof(int));
+  int *nl = malloc(sizeof(int) * 2 * (size_t)(np > 0 ? np : 1));
   long k;
+  int g;
   for (k = 0; k < np; k++) cnt[pairs[2 * k] + 1]++;
   for (g = 0; g < ngroup; g++) cnt[g + 1] += cnt[g];
-  int *nl = malloc(sizeof(int) * 2 * (size_t)(np > 0 ? np : 1));
   for (k = 0; k < np; k++) {
     const int row = pairs[2 * k];
     const int pos = cnt[row]++;
     nl[2 * pos] = row;
     nl[2 * pos + 1] = pairs[2 * k + 1];
   }
-  free(cnt); free(pairs);
+  free(cnt);
+  return nl;
+}
+
+static int *build_nl(const orc_params_t *p, int ngroup, const int *group, const double *r,
+                     long *np_out)
+{
+  double *gr = malloc(sizeof(double) * 3 * (size_t)ngroup);
+  int g;
+  for (g = 0; g < ngroup; g++) {
+    gr[3 * g] = r[3 * group[2 * g]];
+    gr[3 * g + 1] = r[3 * group[2 * g] + 1];
+    gr[3 * g + 2] = r[3 * group[2 * g] + 2];
+  }
+  long np = orc_nl_build(ngroup, gr, p->boxl, p->rc, p->skin, NULL, 0);
+  if (np < 0) { free(gr); return NULL; }
+  int *pairs = malloc(sizeof(int) * 2 * (size_t)(np > 0 ? np : 1));
+  orc_nl_build(ngroup, gr, p->boxl, p->rc, p->skin, pairs, np);
+  int *nl = nl_rows(ngroup, np, pairs);
+  free(pairs);
+  free(gr);
+  *np_out = np;
+  return nl;
+}
+
+/* eleclj.cpp:785-807 + 912-959 for a cubic box with a given neighbor list */
+static int evaluate_nl(const orc_params_t *p, int nsite, int ngroup, const int *group,
+                       const double *r, const double *q, const double *sig, const double *eps,
+                       const int *site_mask, int nexcl, const int *excl, long np, const int *nl,
+                       double *f, double *energies, double *virial)
+{
+  const double L = p->boxl, V = L * L * L;
+  int i;
   double *phi = calloc((size_t)nsite, sizeof(double));
   double *ev = calloc(3 * (size_t)nsite, sizeof(double));
   double *ljf = calloc(3 * (size_t)nsite, sizeof(double));
@@ -1234,8 +1251,23 @@ int orc_evaluate(const orc_params_t *p, int nsite, int ngroup, const int *group,
   energies[1] = lje + tail;
   energies[2] = tail;
   energies[3] = ue + lje + tail;
-  free(gr); free(nl); free(phi); free(ev); free(ljf); free(cheb);
+  free(phi); free(ev); free(ljf); free(cheb);
   return 0;
+}
+
+/* eleclj.cpp:785-807 with a freshly built neighbor list */
+int orc_evaluate(const orc_params_t *p, int nsite, int ngroup, const int *group,
+                 const double *r, const double *q, const double *sig, const double *eps,
+                 const int *site_mask, int nexcl, const int *excl,
+                 double *f, double *energies, double *virial)
+{
+  long np = 0;
+  int *nl = build_nl(p, ngroup, group, r, &np);
+  if (!nl) return -1;
+  int rc = evaluate_nl(p, nsite, ngroup, group, r, q, sig, eps, site_mask, nexcl, excl, np, nl,
+                       f, energies, virial);
+  free(nl);
+  return rc;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1397,7 +1429,9 @@ void orc_constrain_velocities(int nwater, const double *r, double *v, const doub
   }
 }
 
-/* mdyn.cpp:120-160: velocity Verlet with position and velocity constraints.
+/* mdyn.cpp:120-160: velocity Verlet with position and velocity constraints;
+ * the neighbor list is kept and rebuilt when the skin test of
+ * NeighborList::coordinates_changed fires (nbrlist.cpp:51-77).
  * All groups must be waters (3 sites, O first).  energies_out receives, per
  * step, {potential, kinetic}. */
 int orc_md_run(const orc_params_t *p, int nsite, int ngroup, const int *group,
@@ -1409,23 +1443,38 @@ int orc_md_run(const orc_params_t *p, int nsite, int ngroup, const int *group,
   const int nw = ngroup;
   double *f = malloc(sizeof(double) * 3 * (size_t)nsite);
   double *r0 = malloc(sizeof(double) * 3 * (size_t)nsite);
+  double *gr = malloc(sizeof(double) * 3 * (size_t)ngroup);
+  double *last = malloc(sizeof(double) * 3 * (size_t)ngroup);
   double en[4];
-  int i, s;
-  (void)group;
+  int i, s, g;
+  long np = 0;
+  int *nl = NULL;
+#define FIRST_SITES(dst)                                  \
+  for (g = 0; g < ngroup; g++) {                          \
+    dst[3 * g] = r[3 * group[2 * g]];                     \
+    dst[3 * g + 1] = r[3 * group[2 * g] + 1];             \
+    dst[3 * g + 2] = r[3 * group[2 * g] + 2];             \
+  }
   memcpy(r0, r, sizeof(double) * 3 * (size_t)nsite);
   orc_constrain_positions(nw, r, v, r0, mass, 0, bond, angle_deg, tol, 10000);
   memcpy(r0, r, sizeof(double) * 3 * (size_t)nsite);
-  if (orc_evaluate(p, nsite, ngroup, group, r, q, sig, eps, site_mask, nexcl, excl, f, en, NULL)) {
-    free(f); free(r0); return 0;
-  }
+  nl = build_nl(p, ngroup, group, r, &np);
+  FIRST_SITES(last);
+  evaluate_nl(p, nsite, ngroup, group, r, q, sig, eps, site_mask, nexcl, excl, np, nl, f, en, NULL);
   for (s = 0; s < nsteps; s++) {
     for (i = 0; i < 3 * nsite; i++) v[i] += f[i] * (dt / 2 / mass[i / 3]);
     for (i = 0; i < 3 * nsite; i++) r[i] += v[i] * dt;
     if (orc_constrain_positions(nw, r, v, r0, mass, dt, bond, angle_deg, tol, 10000)) {
-      free(f); free(r0); return -s;
+      free(f); free(r0); free(gr); free(last); free(nl); return -s;
     }
     memcpy(r0, r, sizeof(double) * 3 * (size_t)nsite);
-    orc_evaluate(p, nsite, ngroup, group, r, q, sig, eps, site_mask, nexcl, excl, f, en, NULL);
+    FIRST_SITES(gr);
+    if (orc_nl_needs_rebuild(ngroup, gr, last, p->skin)) {
+      free(nl);
+      nl = build_nl(p, ngroup, group, r, &np);
+      memcpy(last, gr, sizeof(double) * 3 * (size_t)ngroup);
+    }
+    evaluate_nl(p, nsite, ngroup, group, r, q, sig, eps, site_mask, nexcl, excl, np, nl, f, en, NULL);
     for (i = 0; i < 3 * nsite; i++) v[i] += f[i] * (dt / 2 / mass[i / 3]);
     orc_constrain_velocities(nw, r, v, mass, bond, angle_deg);
     if (energies_out) {
@@ -1435,6 +1484,7 @@ int orc_md_run(const orc_params_t *p, int nsite, int ngroup, const int *group,
       energies_out[2 * s + 1] = ke / 2;
     }
   }
-  free(f); free(r0);
+#undef FIRST_SITES
+  free(f); free(r0); free(gr); free(last); free(nl);
   return nsteps;
 }

@@ -264,3 +264,21 @@ def ref_cheby(rc, eta):
     t = np.zeros(3 + 512 * 7)
     ref().ref_cheby(rc, eta, _ptr(t))
     return t
+
+
+def md_run(system, p, nsteps, dt=0.001, r=None, v=None, tol=1e-8):
+    """Oracle velocity-Verlet MD with rigid-water constraints; returns
+    (r, v, energies[nsteps, 2] = (potential, kinetic))."""
+    lib = oracle()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64).copy()
+    v = np.ascontiguousarray(system.v if v is None else v, dtype=np.float64).copy()
+    en = np.zeros((max(nsteps, 1), 2))
+    par = params(system, p)
+    done = lib.orc_md_run(C.byref(par), system.nsite, system.ngroup,
+                          _ptr(np.ascontiguousarray(system.group)), _ptr(r), _ptr(v),
+                          _ptr(system.mass), _ptr(system.q), _ptr(system.sig), _ptr(system.eps),
+                          _ptr(system.site_mask), len(system.excl),
+                          _ptr(np.ascontiguousarray(system.excl)), dt, nsteps, 1.0, 109.47, tol,
+                          _ptr(en))
+    assert done == nsteps, done
+    return r, v, en[:nsteps]

@@ -1,0 +1,412 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each oracle function is checked against the reference's own outputs (golden
+fixtures written by tests/golden/make_golden.py from the compiled reference
+sources, and, where it is built here, the live reference harness) and
+against what the mathematics fixes: brute-force enumeration, closed forms,
+invariants and textbook constants.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from molmodel_amd import waterbox as W
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+# ---------------------------------------------------------------- PairBinary
+def test_pbin_kat_matches_reference_expected_output():
+    d = json.load(open(os.path.join(GOLD, "pbin_kat.json")))
+    a = np.array(d["adds"], dtype=np.int32).reshape(-1)
+    rowptr = np.zeros(11, dtype=np.int32)
+    cols = np.zeros(len(d["adds"]), dtype=np.int32)
+    nnz = O.oracle().orc_pbin_build(10, len(d["adds"]), O._ptr(a), O._ptr(rowptr), O._ptr(cols))
+    rows = [cols[rowptr[i]:rowptr[i + 1]].tolist() for i in range(10)]
+    assert rows == d["expected_rows_from_comment"] == d["rows"]
+    assert nnz == sum(len(r) for r in rows)
+    ex = [[O.oracle().orc_pbin_exists(O._ptr(rowptr), O._ptr(cols), i, j) for j in range(10)]
+          for i in range(10)]
+    assert ex == d["exists"]
+
+
+def test_canonical_rule():
+    # even i+j -> row max(i,j); odd -> row min(i,j)
+    for i in range(12):
+        for j in range(12):
+            a, b = O.C.c_int(i), O.C.c_int(j)
+            O.oracle().orc_pbin_canonical(O.C.byref(a), O.C.byref(b))
+            row = a.value
+            assert row == (max(i, j) if (i + j) % 2 == 0 else min(i, j))
+            assert {a.value, b.value} == {i, j}
+
+
+# ---------------------------------------------------------------- neighbor list
+def test_nl_matches_reference_fixture_water648():
+    g = gold("ref_water648.npz")
+    s, p = W.config("water648_ewald")
+    assert str(g["r_digest"][0]) == __import__("hashlib").sha256(np.ascontiguousarray(s.r).tobytes()).hexdigest()
+    mine = O.nl_pairs(O.first_sites(s), s.boxl, p["rc"], p["skin"])
+    assert np.array_equal(O.pair_set(mine), O.pair_set(g["nl"]))
+    # same rows and same order within each row as the reference appends them
+    order = np.argsort(mine[:, 0], kind="stable")
+    assert np.array_equal(mine[order], g["nl"])
+
+
+def test_nl_matches_reference_fixture_small_boxes():
+    g = gold("ref_nl_small.npz")
+    i = 0
+    while f"c{i}_n" in g:
+        x, boxl, rc, skin = g[f"c{i}_x"], float(g[f"c{i}_boxl"]), float(g[f"c{i}_rc"]), float(g[f"c{i}_skin"])
+        mine = O.nl_pairs(x, boxl, rc, skin)
+        order = np.argsort(mine[:, 0], kind="stable")
+        assert np.array_equal(mine[order], g[f"c{i}_pairs"]), i
+        i += 1
+    assert i >= 5
+
+
+def test_nl_skin_test_matches_reference_two_step():
+    g = gold("ref_nl_small.npz")
+    n, boxl, rc, skin = g["two_params"]
+    n = int(n)
+    x0 = g["two_x0"]
+    for tag in ("small", "large"):
+        x1 = g[f"two_{tag}_x1"]
+        rebuild = O.oracle().orc_nl_needs_rebuild(n, O._ptr(x1), O._ptr(np.ascontiguousarray(x0)), skin)
+        expect = O.nl_pairs(x1 if rebuild else x0, boxl, rc, skin)
+        assert np.array_equal(O.pair_set(expect), O.pair_set(g[f"two_{tag}_pairs"])), tag
+    assert O.oracle().orc_nl_needs_rebuild(n, O._ptr(g["two_large_x1"]), O._ptr(x0), skin) == 1
+    assert O.oracle().orc_nl_needs_rebuild(n, O._ptr(g["two_small_x1"]), O._ptr(x0), skin) == 0
+
+
+def brute_pairs(x, boxl, c2):
+    d = x[:, None, :] - x[None, :, :]
+    d -= boxl * np.floor(d / boxl + 0.5)
+    r2 = (d * d).sum(-1)
+    i, j = np.nonzero(np.triu(r2 < c2, 1))
+    return set(zip(i.tolist(), j.tolist()))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_nl_characterisation_brute_force(seed):
+    """Set = same-cell pairs U (cell-neighbour pairs with d^2 < (rc+skin)^2),
+    which contains every pair closer than rc+skin."""
+    n, boxl, rc, skin = 300, 24.0, 6.0, 1.5
+    x = W.uniform(seed, 5, 3 * n).reshape(n, 3) * boxl * 1.2 - 0.1 * boxl
+    got = {tuple(sorted(t)) for t in O.nl_pairs(x, boxl, rc, skin).tolist()}
+    within = brute_pairs(x, boxl, (rc + skin) ** 2)
+    assert within <= got
+    ncell = O.oracle().orc_nl_ncell(n)
+    lc = boxl * (1.0 / ncell)
+    cell = np.mod(np.floor(x / lc + 0.5).astype(int), ncell)
+    same = {(i, j) for i in range(n) for j in range(i + 1, n) if (cell[i] == cell[j]).all()}
+    assert got == within | same
+    # no duplicates, no self pairs
+    raw = O.nl_pairs(x, boxl, rc, skin)
+    assert len(raw) == len(got) and (raw[:, 0] != raw[:, 1]).all()
+
+
+def test_ncell_nmax_closed_form():
+    lib = O.oracle()
+    for G, nc in [(216, 4), (4096, 12), (32768, 25), (1040, 8), (1, 1)]:
+        assert lib.orc_nl_ncell(G) == nc
+    # nmax = min(ceil((rc + skin + sqrt(3) lc)/lc), ncell/2+1)
+    assert lib.orc_nl_nmax(4, 18.64, 9.0, 0.30136) == 3
+    assert lib.orc_nl_nmax(25, 99.4, 10.0, 2.0) == 5
+    assert lib.orc_nl_nmax(12, 49.7, 10.0, 2.0) == 5
+
+
+# ---------------------------------------------------------------- Chebyshev
+def test_cheby_matches_reference_table():
+    g = gold("ref_water648.npz")
+    t = O.cheby(9.0, 0.35)
+    assert np.array_equal(t, g["cheby"])
+
+
+def test_cheby_accuracy_vs_erfc():
+    from scipy.special import erfc
+    rc, eta = 9.0, 0.35
+    t = O.cheby(rc, eta)
+    r2 = np.linspace(4.001, 1.2 * rc * rc - 1e-3, 2000)
+    b0 = np.zeros(1); b1 = np.zeros(1)
+    err0 = err1 = 0.0
+    for x in r2:
+        O.oracle().orc_cheby_eval(O._ptr(t), x, O._ptr(b0), O._ptr(b1))
+        r = math.sqrt(x)
+        f = erfc(eta * r) / r
+        df = (f + eta * 2 / math.sqrt(math.pi) * math.exp(-(eta * r) ** 2)) / x
+        err0 = max(err0, abs(b0[0] - f) / f)
+        err1 = max(err1, abs(b1[0] - df) / df)
+    # b1 is -2 d/dr^2 of the fitted cubic itself (cheby.c:30-32), a less
+    # accurate approximation of the exact derivative than b0 is of f
+    assert err0 < 1e-7 and err1 < 5e-5
+
+
+# ---------------------------------------------------------------- real space
+def test_rspace_matches_reference_fixture():
+    g = gold("ref_water648.npz")
+    s, p = W.config("water648_ewald")
+    phi, ev, ljf, lje = O.rspace_parts(s, p)
+    assert np.array_equal(phi, g["phi"])
+    assert np.array_equal(ev, g["evec"])
+    assert np.array_equal(ljf, g["lj_force"])
+    assert lje == g["lj_energy"][0]
+
+
+@pytest.mark.skipif(O.ref() is None, reason="reference sources not built in this environment")
+def test_rspace_matches_live_reference_other_seed():
+    s, p = W.config("water648_ewald", seed=77)
+    a = O.rspace_parts(s, p)
+    b = O.ref_rspace_parts(s, p)
+    for x, y in zip(a, b):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
+def test_forces_are_minus_energy_gradient():
+    """Finite differences of the total energy (oracle) reproduce its forces."""
+    s, p = W.config("water27_small")
+    f, en = O.evaluate(s, p)
+    h = 1e-5
+    rng = np.random.default_rng(0)
+    for k in rng.choice(s.nsite, 4, replace=False):
+        for c in range(3):
+            r1 = s.r.copy(); r1[k, c] += h
+            r2 = s.r.copy(); r2[k, c] -= h
+            e1 = O.evaluate(s, p, r=r1)[1][3]
+            e2 = O.evaluate(s, p, r=r2)[1][3]
+            fd = -(e1 - e2) / (2 * h)
+            assert abs(fd - f[k, c]) <= 2e-5 * max(1.0, abs(f[k, c])), (k, c, fd, f[k, c])
+
+
+def test_newton_third_law_real_space():
+    s, p = W.config("water648_ewald")
+    phi, ev, ljf, lje = O.rspace_parts(s, p)
+    ftot = (s.q[:, None] * ev + ljf).sum(0)
+    assert np.abs(ftot).max() < 1e-9
+
+
+# ---------------------------------------------------------------- k space
+def test_kspace_matches_reference_fixture():
+    g = gold("ref_water648.npz")
+    s, p = W.config("water648_ewald")
+    phi, ev, vir, en = O.kspace_parts(s, p["kc"], p["eta"])
+    assert np.array_equal(phi, g["kphi"])
+    assert np.array_equal(ev, g["kevec"])
+    assert O.oracle().orc_kspace_count(7.0) == int(g["nk"][0]) == 709
+
+
+def nacl_system(ncell=2, a=5.64):
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    pos, q = [], []
+    for i in range(ncell):
+        for j in range(ncell):
+            for k in range(ncell):
+                o = np.array([i, j, k], dtype=float)
+                for b in base:
+                    pos.append((o + b) * a); q.append(1.0)
+                    pos.append((o + b + [0.5, 0, 0]) * a); q.append(-1.0)
+    n = len(q)
+    r = np.array(pos)
+    qq = W.e_to_charge_unit(np.array(q))
+    z = np.zeros(n)
+    sys_ = W.System(boxl=ncell * a, r=r, v=np.zeros_like(r), q=qq, sig=z, eps=z,
+                    site_mask=np.full(n, W.SITE_HAS_Q, dtype=np.int32), mass=np.ones(n),
+                    group=np.stack([np.arange(n), np.arange(n) + 1], 1).astype(np.int32),
+                    excl=np.zeros((0, 2), dtype=np.int32))
+    return sys_, a
+
+
+@pytest.mark.parametrize("eta", [0.65, 0.75])
+def test_ewald_madelung_nacl(eta):
+    """Total Ewald energy of rock salt = -M N q^2/(2 r0), M = 1.747565."""
+    s, a = nacl_system()
+    rc = 0.49 * s.boxl
+    p = dict(rc=rc, smoothing_width=0.0, eta=eta, kmode=0, kc=12.0, skin=0.1,
+             include_lj_correction=0)
+    f, en = O.evaluate(s, p)
+    r0 = a / 2
+    expect = -1.747565 * s.nsite * W.CHARGE_UNIT ** 2 / (2 * r0)
+    assert abs(en[0] - expect) < 2e-6 * abs(expect)
+    assert np.abs(f).max() < 1e-6 * W.CHARGE_UNIT ** 2
+
+
+def test_ewald_energy_independent_of_eta():
+    # the group cutoff drops site pairs as close as rc - 2 A, so eta must
+    # make erfc(eta (rc - 2)) negligible; kc must converge the k sum
+    s, p = W.config("water648_ewald")
+    p0 = dict(p, smoothing_width=0.0, kc=17.0, include_lj_correction=0)
+    e = [O.evaluate(s, dict(p0, eta=eta))[1][0] for eta in (0.62, 0.68)]
+    assert abs(e[0] - e[1]) < 5e-6 * abs(e[0])
+
+
+# ---------------------------------------------------------------- P3M
+def test_bspline_weights_are_box_convolutions():
+    """W_p is the p-fold convolution of the unit box (brute force on a grid)."""
+    h = 1e-3
+    x = np.arange(-4, 4, h) + h / 2
+    box = (np.abs(x) < 0.5).astype(float)
+    conv = box.copy()
+    for order in range(1, 8):
+        if order > 1:
+            conv = np.convolve(conv, box, mode="same") * h
+        for s in (-0.37, 0.0, 0.21, 0.45):
+            w = np.zeros(order); d = np.zeros(order)
+            O.oracle().orc_p3m_weights(s, order, O._ptr(w), O._ptr(d))
+            istart = -((order + 1) // 2 - 1)
+            # distance from the particle to grid point istart+k: for odd
+            # orders s is measured from the nearest point, for even orders
+            # from the cell centre
+            off = 0.0 if order % 2 else 0.5
+            for k in range(order):
+                dist = s + off - (istart + k)
+                val = np.interp(dist, x, conv)
+                assert abs(w[k] - val) < 5e-3, (order, s, k)
+            assert abs(w.sum() - 1) < 1e-14
+            assert abs(d.sum()) < 1e-13
+
+
+def test_fft_matches_numpy():
+    rng = np.random.default_rng(5)
+    for shape in [(8, 6, 10), (7, 5, 9), (12, 16, 4)]:
+        re = rng.standard_normal(shape); im = rng.standard_normal(shape)
+        a, b = re.copy(), im.copy()
+        O.oracle().orc_fft3d(*shape, O._ptr(a), O._ptr(b), -1)
+        ref = np.fft.fftn(re + 1j * im)
+        assert np.abs(a + 1j * b - ref).max() < 1e-10 * np.abs(ref).max()
+        O.oracle().orc_fft3d(*shape, O._ptr(a), O._ptr(b), +1)
+        assert np.abs(a + 1j * b - (re + 1j * im) * np.prod(shape)).max() < 1e-9 * np.prod(shape)
+
+
+def test_p3m_grid_sizes():
+    lib = O.oracle()
+    assert lib.orc_p3m_grid_size(18.64, 0.5) == 36     # floor(37.78) = 37 -> 36
+    assert lib.orc_p3m_grid_size(49.7, 1.0354) == 48
+    assert lib.orc_p3m_grid_size(99.4, 1.5531) == 64
+
+
+def test_p3m_converges_to_ewald():
+    """P3M field -> plain Ewald reciprocal field as mesh and order grow."""
+    s, p = W.config("water27_small")
+    kphi, kev, _, ke = O.kspace_parts(s, 12.0, p["eta"])
+    errs = []
+    for mesh, order in [((16, 16, 16), 3), ((24, 24, 24), 5), ((32, 32, 32), 7)]:
+        pp = dict(p, mesh=mesh, order=order, nalias=2, ik=0)
+        phi, ev, _, en = O.p3m_parts(s, pp)
+        errs.append(np.linalg.norm(ev - kev) / np.linalg.norm(kev))
+    assert errs[2] < errs[1] < errs[0]
+    assert errs[2] < 1e-4
+
+
+def test_p3m_green_function_properties():
+    g, chi = O.green((16, 16, 16), 20.0, 0.35, 5)
+    assert g[0, 0, 0] == 0
+    # inversion symmetry across the half spectrum: G(-k) = G(k)
+    for ix in range(1, 16):
+        for iy in range(1, 16):
+            assert abs(g[ix, iy, 0] - g[16 - ix, 16 - iy, 0]) < 1e-12 * abs(g[ix, iy, 0]) + 1e-300
+    # small k: the aliases vanish and G -> g(k)/U(k)^2, with
+    # g = 4 pi/(V k^2) exp(-k^2/4 eta^2) and U = sinc(pi j/n)^order
+    k = 2 * np.pi / 20.0
+    cont = 4 * np.pi / (20.0 ** 3 * k * k) * math.exp(-k * k / (4 * 0.35 ** 2))
+    u2 = (math.sin(math.pi / 16) / (math.pi / 16)) ** 10
+    assert abs(g[1, 0, 0] - cont / u2) < 1e-6 * cont
+    assert 0 < chi < 1
+
+
+def test_p3m_energy_equals_half_q_phi():
+    s, p = W.config("water27_small")
+    pp = dict(p, mesh=(16, 16, 16), order=5, nalias=2, ik=0)
+    phi, ev, vir, en = O.p3m_parts(s, pp)
+    assert abs(en - 0.5 * np.dot(s.q, phi)) < 1e-9 * abs(en)
+
+
+def test_p3m_ik_and_ad_agree():
+    s, p = W.config("water27_small")
+    base = dict(p, mesh=(24, 24, 24), order=5, nalias=2)
+    _, ev_ad, _, _ = O.p3m_parts(s, dict(base, ik=0))
+    _, ev_ik, _, _ = O.p3m_parts(s, dict(base, ik=1))
+    assert np.linalg.norm(ev_ad - ev_ik) / np.linalg.norm(ev_ik) < 2e-3
+
+
+# ---------------------------------------------------------------- ElecLJ terms
+def test_self_and_background_closed_form():
+    q = np.array([1.0, -0.5, 0.25])
+    phi = np.zeros(3)
+    eta, V = 0.3, 1000.0
+    O.oracle().orc_self_phi(3, O._ptr(q), eta, V, O._ptr(phi))
+    expect = -eta * 2 / math.sqrt(math.pi) * q - math.pi / (eta ** 2 * V) * q.sum()
+    assert np.allclose(phi, expect, rtol=1e-15, atol=1e-15)
+
+
+def test_lj_tail_sharp_cutoff_limit():
+    """As the smoothing width -> 0 the tail tends to the textbook
+    2 pi/V sum_ij 4 eps sig^6 (sig^6/(9 rc^9) - 1/(3 rc^3))."""
+    s, p = W.config("water648_ewald")
+    V = s.boxl ** 3
+    lib = O.oracle()
+    w = 1e-4
+    t = lib.orc_lj_tail(s.nsite, O._ptr(s.site_mask), O._ptr(s.sig), O._ptr(s.eps), 0, 9.0, w, V)
+    n = s.nwater
+    sig, eps = 3.166, 0.1553
+    rc = 9.0 - w / 2
+    expect = 2 * math.pi / V * n * n * 4 * eps * sig ** 6 * (sig ** 6 / (9 * rc ** 9) - 1 / (3 * rc ** 3))
+    assert abs(t - expect) < 1e-4 * abs(expect)
+
+
+def test_units():
+    assert abs(W.CHARGE_UNIT ** 2 - 332.06371) < 1e-4
+    assert abs(O.oracle().orc_e_to_charge_unit.__call__ is not None)
+
+
+# ---------------------------------------------------------------- virial
+@pytest.mark.parametrize("name", ["water27_small"])
+def test_virial_trace_equals_scaling_derivative(name):
+    """-dU/dlambda at lambda=1 for r -> lambda r, L -> lambda L equals tr W
+    (Ewald: real space + k space + excluded + tail)."""
+    s, p = W.config(name)
+    p = dict(p, kc=8.0)
+    f, en, w = O.evaluate(s, p, virial=True)
+    h = 1e-6
+    def U(lam):
+        s2 = W.System(**{**s.__dict__, "boxl": s.boxl * lam, "r": s.r * lam})
+        # the table and eta stay fixed; keep rc fixed as well
+        return O.evaluate(s2, p)[1][3]
+    dU = (U(1 + h) - U(1 - h)) / (2 * h)
+    assert abs(-dU - (w[0] + w[4] + w[8])) < 2e-4 * max(1.0, abs(dU))
+
+
+# ---------------------------------------------------------------- dynamics
+def test_constraint_iteration_restores_geometry():
+    s, p = W.config("water27_small")
+    r = s.r + 0.01 * (W.uniform(3, 1, 3 * s.nsite).reshape(-1, 3) - 0.5)
+    v = s.v.copy()
+    lib = O.oracle()
+    r0 = np.ascontiguousarray(s.r)
+    fails = lib.orc_constrain_positions(s.nwater, O._ptr(r), O._ptr(v), O._ptr(r0), O._ptr(s.mass),
+                                        0.001, 1.0, 109.47, 1e-10, 1000)
+    assert fails == 0
+    x = r.reshape(-1, 3, 3)
+    a, b = x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]
+    assert np.abs(np.linalg.norm(a, axis=1) - 1).max() < 1e-9
+    cos = (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1))
+    assert np.abs(np.arccos(cos) - np.radians(109.47)).max() < 1e-9
+    # the correction lies in M^-1 span{grad sigma(r0)}: centre of mass unchanged
+    m = s.mass.reshape(-1, 3)
+    com0 = (m[..., None] * (s.r + 0.01 * (W.uniform(3, 1, 3 * s.nsite).reshape(-1, 3) - 0.5)).reshape(-1, 3, 3)).sum(1)
+    com1 = (m[..., None] * x).sum(1)
+    assert np.abs(com0 - com1).max() < 1e-12
+
+
+def test_md_energy_conservation_oracle():
+    s, p = W.config("water27_small")
+    r, v, en = O.md_run(s, p, 60)
+    etot = en[:, 0] + en[:, 1]
+    assert np.abs(etot - etot[0]).max() < 0.02 * en[:, 1].mean()
