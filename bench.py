@@ -35,12 +35,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--serial-kspace", action="store_true",
+                    help="run the reciprocal-space kernels on the main stream")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device copy of this many MB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "the multi-GPU design is documented but not built", file=sys.stderr)
         sys.exit(2)
 
+    if args.serial_kspace:
+        os.environ["MOLMODEL_AMD_SERIAL_KSPACE"] = "1"
     import numpy as np
     from molmodel_amd import waterbox as W
     from molmodel_amd.engine import Engine, lib
@@ -70,6 +76,10 @@ def main():
     ns_day = (1e3 / ms_step) * (dt_ps * 1e3) * 86400 / 1e6
     en = e.energies()[0]
 
+    if args.copy_probe_mb:
+        ms_c = C.c_double(0)
+        lib().mm_copy_probe.argtypes = [C.c_long, C.c_void_p]
+        assert lib().mm_copy_probe(args.copy_probe_mb << 20, C.byref(ms_c)) == 0
     # pair kernel: HIP events on the engine stream
     ms = C.c_double(0)
     assert lib().mm_time_kernel(e.h, 0, 50, C.byref(ms)) == 0

@@ -123,7 +123,7 @@ int mm_kspace_field(mm_engine *e, double *phi, double *evec);
  * canonical (row, col) group pairs (pbin.c:6) in that system's group
  * numbering; returns the pair count (writes at most maxp pairs). */
 long mm_nl_pairs(mm_engine *e, int sys, int *pairs, long maxp);
-long long mm_nl_builds(mm_engine *e);
+long long mm_nl_builds(mm_engine *e);   /* neighbour-list builds so far, all systems */
 
 /* MD with velocity Verlet and rigid-water constraints (mdyn.cpp:120-160):
  * setup constrains the initial positions (constraint tolerance tol) and
@@ -131,6 +131,59 @@ long long mm_nl_builds(mm_engine *e);
  * replays each step as one captured hipGraph. */
 int mm_md_setup(mm_engine *e, double dt, double bond, double angle_deg, double tol);
 int mm_md_run(mm_engine *e, int nsteps, int use_graph);
+
+/* Replica exchange among the engine's systems (mdyn.cpp:317-351, 476-497).
+ * ladder: system indices by ascending temperature, ladders separated by -1
+ * (systems on one ladder must be copies of one system: same sites, groups,
+ * box and k-space); kT: nsys values in kcal/mol.  Requires mm_md_setup.
+ * run: nintervals x (steps MD steps (0: none), then, if reinit_velocities, new
+ * Maxwell-Boltzmann velocities at each system's kT, then one exchange
+ * attempt per neighbouring pair from the hottest down, positions swapped,
+ * forces recomputed).  stats: tries and accepts per system (as the lower
+ * member of a pair), 2 per system; config: which starting configuration
+ * each system now holds. */
+int mm_remd_setup(mm_engine *e, int nladder, const int *ladder, const double *kT,
+                  unsigned long long seed);
+int mm_remd_run(mm_engine *e, int nintervals, int steps, int reinit_velocities, int use_graph);
+int mm_remd_stats(mm_engine *e, long long *tries_accepts, int *config);
+
+/* Reciprocal-space parameters for one cubic system chosen by this library's
+ * tuner (no reference counterpart: the reference takes them from the user
+ * or optimises eta for a fixed mesh, pppm.c:667-716).  eta from the
+ * real-space error estimate at tol/sqrt(2) (eleclj.cpp:134-138); then the
+ * cheapest of Ewald (smallest kc within tol/sqrt(2), eleclj.cpp:128-132)
+ * and P3M (per order, the smallest 2357-smooth cubic mesh whose chi,
+ * pppm.c:313-393, is within tol/sqrt(2)) under a device cost model.
+ * Host-only: needs no GPU. */
+typedef struct {
+  int kmode;            /* 0 Ewald, 1 P3M */
+  double eta, kc;
+  int mesh[3], order;
+  double err_rspace, err_kspace;   /* the estimates at the choice */
+  double cost;                     /* modelled device time, microseconds */
+} mm_kspace_choice;
+int mm_tune_kspace(double boxl, int nsite, double rc, double tol, int allow_ewald,
+                   mm_kspace_choice *out);
+
+/* Gradients of system `sys`'s potential energy and of the force-matching
+ * loss L = sum_i |F_i - f_ref_i|^2 with respect to per-type parameters
+ * (no reference counterpart; the nearest is ElecLJ::set_lambda's
+ * parameter interpolation, eleclj.cpp:273-319).  Site types are the
+ * distinct (q, sigma, epsilon, mask) of the system's sites, numbered in
+ * order of first appearance among the sites; the per-type parameters are
+ * (q [internal charge units], sigma, epsilon).  Forces are bilinear in the
+ * charges and linear in each LJ type pair's a = 4 e s^12, b = 4 e s^6, so
+ * the gradients come from a small basis of device force evaluations at
+ * the current positions (pair terms recomputed); sigma and epsilon through
+ * the mixing rule on the host.  Outputs (any may be NULL except ntype):
+ * ntype; site_type[nsite]; type_params[3*ntype]; dE, dL [3*ntype] ordered
+ * (dq, dsigma, depsilon) per type; loss; basis_err = relative RMS
+ * difference between the forces reassembled from the basis and the
+ * evaluated forces.  f_ref may be NULL (energy gradients only).  Leaves
+ * the engine's parameters and forces as they were. */
+int mm_param_gradients(mm_engine *e, int sys, const double *f_ref, int maxtypes, int *ntype,
+                       int *site_type, double *type_params, double *dE, double *dL,
+                       double *loss, double *basis_err);
 
 /* interaction counts of the current neighbour list, 3 per system:
  * group pairs with first-site r^2 <= rc^2 (unordered), charged site pairs
@@ -143,6 +196,10 @@ int mm_count_pairs(mm_engine *e, long long *counts);
 #define MM_KERNEL_PAIR 0
 #define MM_KERNEL_P3M 1
 int mm_time_kernel(mm_engine *e, int which, int reps, double *ms_per_launch);
+
+/* device-to-device copy of `bytes` between two fresh buffers (a reference
+ * kernel of known traffic for calibrating hardware byte counters) */
+int mm_copy_probe(long bytes, double *ms);
 
 /* stream the engine runs on (hipStream_t) and device pointers of the FP64
  * SoA state (x[0..N), y[N..2N), z[2N..3N)) for device-resident callers */

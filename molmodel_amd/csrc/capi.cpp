@@ -1,17 +1,11 @@
 // capi.cpp -- the extern "C" engine API declared in include/molmodel_amd.h.
 #include "../../include/molmodel_amd.h"
-#include "engine.hpp"
+#include "handle.hpp"
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
 using namespace mmamd;
-
-struct mm_engine {
-  Engine e;
-  int pos_dirty = 1;
-  int want_virial_last = 0;
-};
 
 #define CHECK_ENGINE(p)                       \
   do {                                        \
@@ -189,6 +183,9 @@ static int grow_nbr(Engine &e) {
   e.nbcap = std::min(e.nbcap * 2, ((e.maxg + 15) / 16) * 16);
   MM_HIP(hipMalloc((void **)&e.d_nbr, sizeof(int) * (size_t)e.ngroup * e.nbcap));
   e.first_eval = 1;
+  // a captured step holds the old row buffer: capture again on next use
+  if (e.gexec) { hipGraphExecDestroy(e.gexec); e.gexec = nullptr; }
+  if (e.graph) { hipGraphDestroy(e.graph); e.graph = nullptr; }
   return 0;
 }
 
@@ -199,6 +196,19 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
     TRY(launch_p3m(e, want_virial));
     return 0;
   }
+  // The reciprocal-space branch reads only positions and charges and writes
+  // its own buffers, so it runs on the second stream beside the neighbour
+  // list and pair kernels; the join comes before anything that consumes it.
+  const bool fork = e.stream2 != nullptr;
+  hipStream_t kst = e.stream;
+  if (fork) {
+    MM_HIP(hipEventRecord(e.ev_fork, e.stream));
+    MM_HIP(hipStreamWaitEvent(e.stream2, e.ev_fork, 0));
+    kst = e.stream2;
+  }
+  TRY(launch_kspace(e, want_virial, kst));
+  TRY(launch_p3m(e, want_virial, kst));
+  if (fork) MM_HIP(hipEventRecord(e.ev_join, e.stream2));
   if (e.external_nl) {
     e.first_eval = 0;
   } else if (e.first_eval) {
@@ -211,8 +221,7 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
   m->pos_dirty = 0;
   TRY(launch_pack(e));
   TRY(launch_pair(e, field_mode, want_virial));
-  TRY(launch_kspace(e, want_virial));
-  TRY(launch_p3m(e, want_virial));
+  if (fork) MM_HIP(hipStreamWaitEvent(e.stream, e.ev_join, 0));
   return 0;
 }
 
@@ -237,6 +246,12 @@ static int compute_checked(mm_engine *m, int field_mode, int want_virial) {
   set_last_error("neighbour list overflow persisted");
   return MM_ERR_DEVICE;
 }
+
+}  // extern "C"
+
+int mmamd::handle_compute(mm_engine *m) { return compute_checked(m, 0, 0); }
+
+extern "C" {
 
 int mm_compute(mm_engine *m, int flags) {
   CHECK_ENGINE(m);
@@ -389,11 +404,9 @@ static int enqueue_step(mm_engine *m, int mode) {
   return 0;
 }
 
-int mm_md_run(mm_engine *m, int nsteps, int use_graph) {
-  CHECK_ENGINE(m);
+// nsteps of velocity Verlet, queued without waiting for the device
+static int md_run_async(mm_engine *m, int nsteps, int use_graph) {
   Engine &e = m->e;
-  if (!e.md_ready) { set_last_error("mm_md_setup first"); return MM_ERR_ARG; }
-  if (nsteps <= 0) return MM_OK;
   if (e.md_phase == 0) {
     // first half of a step with the current forces (mdyn.cpp:144-150)
     TRY(launch_finish_ex(e, FIN_DRIFT, 0, e.d_phi));
@@ -419,6 +432,10 @@ int mm_md_run(mm_engine *m, int nsteps, int use_graph) {
   // last step ends synchronised: r, v, F of the same time
   TRY(enqueue_step(m, FIN_KICK));
   e.md_phase = 0;
+  return 0;
+}
+
+static int md_check_errors(Engine &e) {
   int f = 0;
   TRY(read_err(e, &f));
   if (f) {
@@ -430,6 +447,98 @@ int mm_md_run(mm_engine *m, int nsteps, int use_graph) {
   return MM_OK;
 }
 
+int mm_md_run(mm_engine *m, int nsteps, int use_graph) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  if (!e.md_ready) { set_last_error("mm_md_setup first"); return MM_ERR_ARG; }
+  if (nsteps <= 0) return MM_OK;
+  TRY(md_run_async(m, nsteps, use_graph));
+  return md_check_errors(e);
+}
+
+int mm_remd_setup(mm_engine *m, int nladder, const int *ladder, const double *kT,
+                  unsigned long long seed) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  if (nladder <= 0 || !ladder || !kT) { set_last_error("bad replica-exchange arguments"); return MM_ERR_ARG; }
+  std::vector<int> seen(e.nsys, 0);
+  int prev = -1;
+  for (int k = 0; k < nladder; k++) {
+    const int s = ladder[k];
+    if (s < -1 || s >= e.nsys) { set_last_error("ladder entry out of range"); return MM_ERR_ARG; }
+    if (s == -1) { prev = -1; continue; }
+    if (seen[s]++) { set_last_error("a system appears twice in the ladders"); return MM_ERR_ARG; }
+    if (prev >= 0) {
+      const SysParams &A = e.hsp[prev], &B = e.hsp[s];
+      // replicas exchange coordinates, so they must be copies of one system
+      if (A.nsite != B.nsite || A.ngroup != B.ngroup || A.L != B.L || A.kmode != B.kmode) {
+        set_last_error("systems on one ladder must have the same sites, groups, box and k-space");
+        return MM_ERR_ARG;
+      }
+    }
+    prev = s;
+  }
+  for (int s = 0; s < e.nsys; s++)
+    if (!(kT[s] > 0)) { set_last_error("kT must be positive for every system"); return MM_ERR_ARG; }
+  if (!e.md_ready) { set_last_error("mm_md_setup first"); return MM_ERR_ARG; }
+  auto realloc_d = [](auto **p, size_t n) -> int {
+    if (*p) MM_HIP(hipFree(*p));
+    MM_HIP(hipMalloc((void **)p, n * sizeof(**p)));
+    MM_HIP(hipMemset(*p, 0, n * sizeof(**p)));
+    return 0;
+  };
+  TRY(realloc_d(&e.d_ladder, nladder));
+  TRY(realloc_d(&e.d_kT, e.nsys));
+  TRY(realloc_d(&e.d_tail, e.nsys));
+  TRY(realloc_d(&e.d_perm, 3 * (size_t)e.nsys));
+  TRY(realloc_d(&e.d_remd_stats, 2 * (size_t)e.nsys));
+  TRY(realloc_d(&e.d_rtmp, 3 * (size_t)e.nsite));
+  TRY(realloc_d(&e.d_mom, 4 * (size_t)e.nsys));
+  MM_HIP(hipMemcpy(e.d_ladder, ladder, sizeof(int) * nladder, hipMemcpyHostToDevice));
+  MM_HIP(hipMemcpy(e.d_kT, kT, sizeof(double) * e.nsys, hipMemcpyHostToDevice));
+  MM_HIP(hipMemcpy(e.d_tail, e.lj_tail.data(), sizeof(double) * e.nsys, hipMemcpyHostToDevice));
+  std::vector<int> ids(3 * e.nsys);
+  for (int s = 0; s < e.nsys; s++) ids[s] = ids[e.nsys + s] = ids[2 * e.nsys + s] = s;
+  MM_HIP(hipMemcpy(e.d_perm, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice));
+  e.nladder = nladder;
+  e.remd_seed = seed;
+  e.remd_attempt = 0;
+  return MM_OK;
+}
+
+int mm_remd_run(mm_engine *m, int nintervals, int steps, int reinit_velocities, int use_graph) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  if (!e.d_ladder) { set_last_error("mm_remd_setup first"); return MM_ERR_ARG; }
+  if (nintervals < 0 || steps < 0) { set_last_error("bad interval arguments"); return MM_ERR_ARG; }
+  for (int it = 0; it < nintervals; it++) {
+    if (steps > 0) TRY(md_run_async(m, steps, use_graph));
+    const long long a = e.remd_attempt++;
+    // end of the interval's last step: velocity redraw, then the exchange
+    // attempt (mdyn.cpp:163-167, 185-186)
+    if (reinit_velocities) TRY(launch_remd_velocities(e, a));
+    TRY(launch_remd_exchange(e, a));
+    // forces and energies of the configurations now held; swapped systems
+    // rebuild their neighbour lists first
+    TRY(enqueue_forces(m, 0, 0));
+    TRY(launch_finish_ex(e, FIN_FORCES, 0, e.d_phi));
+  }
+  return md_check_errors(e);
+}
+
+int mm_remd_stats(mm_engine *m, long long *tries_accepts, int *config) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  if (!e.d_ladder) { set_last_error("mm_remd_setup first"); return MM_ERR_ARG; }
+  MM_HIP(hipStreamSynchronize(e.stream));
+  if (tries_accepts)
+    MM_HIP(hipMemcpy(tries_accepts, e.d_remd_stats, sizeof(long long) * 2 * e.nsys,
+                     hipMemcpyDeviceToHost));
+  if (config)
+    MM_HIP(hipMemcpy(config, e.d_perm + e.nsys, sizeof(int) * e.nsys, hipMemcpyDeviceToHost));
+  return MM_OK;
+}
+
 void *mm_stream(mm_engine *m) { return m ? (void *)m->e.stream : nullptr; }
 double *mm_device_positions(mm_engine *m) { return m ? m->e.d_r : nullptr; }
 double *mm_device_forces(mm_engine *m) { return m ? m->e.d_f : nullptr; }
@@ -438,7 +547,18 @@ int mm_sync(mm_engine *m) {
   MM_HIP(hipStreamSynchronize(m->e.stream));
   return MM_OK;
 }
-long long mm_nl_builds(mm_engine *m) { return m ? 0 : -1; }
+long long mm_nl_builds(mm_engine *m) {
+  if (!m) return -1;
+  Engine &e = m->e;
+  if (hipStreamSynchronize(e.stream) != hipSuccess) return -1;
+  std::vector<unsigned long long> b(e.nsys);
+  if (hipMemcpy(b.data(), e.d_nlb, sizeof(unsigned long long) * e.nsys, hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return -1;
+  long long t = 0;
+  for (auto x : b) t += (long long)x;
+  return t;
+}
 
 int mm_count_pairs(mm_engine *m, long long *counts) {
   CHECK_ENGINE(m);
@@ -475,3 +595,25 @@ int mm_time_kernel(mm_engine *m, int which, int reps, double *ms_per_launch) {
 }
 
 }  // extern "C"
+
+extern "C" int mm_copy_probe(long bytes, double *ms) {
+  void *a = nullptr, *b = nullptr;
+  MM_HIP(hipMalloc(&a, bytes));
+  MM_HIP(hipMalloc(&b, bytes));
+  MM_HIP(hipMemset(a, 1, bytes));
+  hipEvent_t e0, e1;
+  MM_HIP(hipEventCreate(&e0));
+  MM_HIP(hipEventCreate(&e1));
+  MM_HIP(hipEventRecord(e0, nullptr));
+  MM_HIP(hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice));
+  MM_HIP(hipEventRecord(e1, nullptr));
+  MM_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  MM_HIP(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  return MM_OK;
+}

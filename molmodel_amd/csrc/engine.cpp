@@ -200,12 +200,6 @@ static bool radix_ok(int n) {
   return n == 1;
 }
 
-static int largest_div_le(int n, int lim) {
-  for (int b = lim; b >= 1; b--)
-    if (n % b == 0) return b;
-  return 1;
-}
-
 template <class T>
 static int dalloc(T **p, size_t n) {
   if (n == 0) n = 1;
@@ -226,6 +220,24 @@ static int dput(T *d, const std::vector<T> &h) {
     int rc_ = (x);         \
     if (rc_) return rc_;   \
   } while (0)
+
+// The tail energy is linear in totA = sum n_a n_b a_ab and
+// totB = -sum n_a n_b b_ab: U = ca totA + cb totB (eleclj.cpp:1259-1264)
+void lj_tail_factors(const SystemInput &s, double *ca, double *cb) {
+  const double V = s.boxl * s.boxl * s.boxl;
+  const double rhi = s.rc, rlo = rhi - s.width;
+  const double rlo2 = rlo * rlo, rlo3 = rlo2 * rlo, rlo4 = rlo3 * rlo;
+  const double rlo5 = rlo4 * rlo, rlo6 = rlo5 * rlo, rlo7 = rlo6 * rlo, rlo9 = rlo7 * rlo2;
+  const double rhi2 = rhi * rhi, rhi3 = rhi2 * rhi, rhi4 = rhi3 * rhi, rhi5 = rhi4 * rhi;
+  const double rhi6 = rhi5 * rhi, rhi7 = rhi6 * rhi;
+  const double w = (rhi - rlo) / (63 * rhi3 * rlo9 * pow(rhi + rlo, 5));
+  *ca = 2 * M_PI / V *
+        (1 / (9 * rlo9) - w * (7 * rhi7 + 42 * rhi6 * rlo + 112 * rhi5 * rlo2 + 182 * rhi4 * rlo3 +
+                               217 * rhi3 * rlo4 + 224 * rhi2 * rlo5 + 192 * rhi * rlo6 + 32 * rlo7));
+  *cb = 2 * M_PI / V *
+        (1 / (3 * rlo3) - w * 3 * rhi3 * rlo6 *
+                              (7 * rhi4 + 42 * rhi3 * rlo + 112 * rhi2 * rlo2 + 150 * rhi * rlo3 + 25 * rlo4));
+}
 
 // eleclj.cpp:279-308 + 1247-1266: LJ tail over distinct "%12.6g" keys
 static double lj_tail(const SystemInput &s) {
@@ -389,6 +401,8 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
     // erfc table
     build_table(P, S.rc, S.eta, tab);
     MM_REQ(P.ntabA + P.ntabB <= 1536, "erfc table too large for the box/cutoff");
+    // the pair kernel selects the segment as the smaller of the two index maps
+    MM_REQ(P.tabAinv > P.tabBinv, "erfc table segments out of order");
     // lattice tensors
     double latv[9] = {S.boxl, 0, 0, 0, S.boxl, 0, 0, 0, S.boxl};
     if (S.has_latv) {
@@ -428,6 +442,7 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
       MM_REQ(S.order >= 1 && S.order <= 7, "assignment order must be 1..7");
       MM_REQ(radix_ok(S.nx) && radix_ok(S.ny) && radix_ok(S.nz),
              "mesh dimensions must be <= 128 with prime factors 2,3,5,7");
+      MM_REQ(S.nx >= 8 && S.ny >= 8 && S.nz >= 8, "mesh dimensions must be >= 8");
       MM_REQ(S.nalias >= 0 && S.nalias <= 7, "alias sum limit must be 0..7");
       P.nx = S.nx; P.ny = S.ny; P.nz = S.nz; P.order = S.order;
       for (int a = 0; a < 9; a++) P.T[a] = inv[a];
@@ -458,8 +473,8 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
           const double a = -2 * M_PI * (t % nn[d]) / nn[d];
           twid.push_back(make_float2((float)cos(a), (float)sin(a)));
         }
-      P.bx = largest_div_le(S.nx, 4);
-      P.by = largest_div_le(S.ny, 4);
+      MM_REQ(choose_spread_blocks(S.nx, S.ny, S.nz, 4096, 4096, P.bx, P.by),
+             "mesh dimensions give no valid spreading column blocks");
       P.nbx = S.nx / P.bx;
       P.nby = S.ny / P.by;
       P.bin0 = bin0;
@@ -486,12 +501,20 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   e.nbcap = (maxnb + 15) / 16 * 16;
   const int N = e.nsite, G = e.ngroup;
   // partial-sum slots: finish blocks and x-pass blocks
-  int nb = (e.maxg + 255) / 256;
+  int nb = (e.maxg + 63) / 64;
   for (auto &P : e.hsp)
-    if (P.kmode == 1) nb = std::max(nb, (P.ny * (P.nz / 2 + 1) + 15) / 16);
+    if (P.kmode == 1)
+      nb = std::max(nb, (P.ny * (P.nz / 2 + 1) + P3M_FFT_LINES - 1) / P3M_FFT_LINES);
   e.nblk_part = nb;
 
   MM_HIP(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+  // MOLMODEL_AMD_SERIAL_KSPACE=1 keeps the reciprocal-space branch on the
+  // main stream (for timing the two arrangements against each other)
+  const char *serial = getenv("MOLMODEL_AMD_SERIAL_KSPACE");
+  if (!(serial && serial[0] == '1'))
+    MM_HIP(hipStreamCreateWithFlags(&e.stream2, hipStreamNonBlocking));
+  MM_HIP(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
+  MM_HIP(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
   MM_TRY(dalloc(&e.d_sp, e.nsys));
   MM_TRY(dput(e.d_sp, e.hsp));
   MM_TRY(dalloc(&e.d_gfirst, G)); MM_TRY(dput(e.d_gfirst, gfirst));
@@ -516,6 +539,7 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   MM_TRY(dalloc(&e.d_nnb, G));
   MM_TRY(dalloc(&e.d_rebuild, e.nsys));
   MM_TRY(dalloc(&e.d_err, 1));
+  MM_TRY(dalloc(&e.d_nlb, e.nsys));
   MM_TRY(dalloc(&e.d_gpos, G));
   MM_TRY(dalloc(&e.d_spk, (size_t)G * MAXS));
   MM_TRY(dalloc(&e.d_gmeta, G));
@@ -536,7 +560,8 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   MM_TRY(dalloc(&e.d_bin_count, e.nbin_total + 1));
   MM_TRY(dalloc(&e.d_bin_start, e.nbin_total + 1));
   MM_TRY(dalloc(&e.d_bin_fill, e.nbin_total + 1));
-  MM_TRY(dalloc(&e.d_bin_sites, N));
+  MM_TRY(dalloc(&e.d_bgrid, N));
+  MM_TRY(dalloc(&e.d_bfrac, N));
   MM_TRY(dalloc(&e.d_acc, (size_t)e.nsys * NACC));
   MM_TRY(dalloc(&e.d_part, (size_t)e.nsys * e.nblk_part * 16));
   MM_TRY(dalloc(&e.d_done, 4 * (size_t)e.nsys));
@@ -559,10 +584,15 @@ void engine_destroy(Engine &e) {
                   e.d_err, e.d_gpos, e.d_spk, e.d_gmeta, e.d_pf, e.d_plj, e.d_gacc, e.d_tab,
                   e.d_ljt, e.d_kres, e.d_kvec, e.d_ksc, e.d_mesh, e.d_cmesh, e.d_green, e.d_twid,
                   e.d_sgrid, e.d_sfrac, e.d_bin_count, e.d_bin_start, e.d_bin_fill,
-                  e.d_bin_sites, e.d_acc, e.d_part, e.d_done, e.d_phi, e.d_scratch};
+                  e.d_bgrid, e.d_bfrac, e.d_acc, e.d_part, e.d_done, e.d_phi, e.d_scratch,
+                  e.d_nlb, e.d_ladder, e.d_kT, e.d_tail, e.d_rtmp, e.d_mom, e.d_perm,
+                  e.d_remd_stats};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (e.stream) hipStreamDestroy(e.stream);
+  if (e.stream2) hipStreamDestroy(e.stream2);
+  if (e.ev_fork) hipEventDestroy(e.ev_fork);
+  if (e.ev_join) hipEventDestroy(e.ev_join);
   e = Engine();
 }
 

@@ -56,6 +56,10 @@ struct Engine {
   std::vector<SysParams> hsp;
   std::vector<double> lj_tail;   // per system (host FP64)
   hipStream_t stream = nullptr;
+  // second stream for the reciprocal-space branch of a force evaluation
+  // (forked from and joined back into `stream` with the two events)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   SysParams *d_sp = nullptr;
   // topology, original order
@@ -103,7 +107,9 @@ struct Engine {
   int4 *d_sgrid = nullptr;    // per site: jx jy jz bin
   float4 *d_sfrac = nullptr;  // per site: sx sy sz q
   int nbin_total = 0;
-  int *d_bin_count = nullptr, *d_bin_start = nullptr, *d_bin_fill = nullptr, *d_bin_sites = nullptr;
+  int *d_bin_count = nullptr, *d_bin_start = nullptr, *d_bin_fill = nullptr;
+  int4 *d_bgrid = nullptr;    // bin-sorted copies of d_sgrid (w = site index) ...
+  float4 *d_bfrac = nullptr;  // ... and of d_sfrac
   // reductions
   double *d_acc = nullptr;    // NACC per system
   double *d_part = nullptr;   // per-block partials
@@ -119,6 +125,16 @@ struct Engine {
   int md_phase = 0;          // 0: r, v, F at the same step; 1: drifted half step ahead
   double *d_phi = nullptr;   // per-site total potential of the last evaluation
   double *d_scratch = nullptr; size_t scratch_n = 0;
+  unsigned long long *d_nlb = nullptr;   // neighbour-list builds per system
+  // replica exchange: ladders of system indices by ascending temperature
+  // (separated by -1), kT per system, per-attempt permutation, which
+  // starting configuration each system holds, tries/accepts per system
+  int *d_ladder = nullptr; int nladder = 0;
+  double *d_kT = nullptr, *d_tail = nullptr, *d_rtmp = nullptr, *d_mom = nullptr;
+  int *d_perm = nullptr;      // 3 * nsys: perm, configuration id, scratch
+  long long *d_remd_stats = nullptr;
+  unsigned long long remd_seed = 0;
+  long long remd_attempt = 0;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -132,18 +148,27 @@ double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl
                 double inv_4eta2, double inv_v, int nalias, int order, double *chi_acc);
 void cheb_fit(double a, double b, double (*f)(double), double *c4);
 void engine_destroy(Engine &e);
+void lj_tail_factors(const SystemInput &s, double *ca, double *cb);
 
 // launchers (each returns 0 or a negative status)
 int launch_nl_build(Engine &e, int force);
 int launch_pack(Engine &e);
 int launch_pair(Engine &e, int field_mode, int want_virial);
-int launch_kspace(Engine &e, int want_virial);
-int launch_p3m(Engine &e, int want_virial);
+// mesh lines per FFT workgroup (16 threads per line); the x pass writes one
+// partial-sum slot per workgroup
+constexpr int P3M_FFT_LINES = 4;
+
+int launch_kspace(Engine &e, int want_virial, hipStream_t st);
+int launch_p3m(Engine &e, int want_virial, hipStream_t st);
+inline int launch_kspace(Engine &e, int want_virial) { return launch_kspace(e, want_virial, e.stream); }
+inline int launch_p3m(Engine &e, int want_virial) { return launch_p3m(e, want_virial, e.stream); }
 int launch_finish(Engine &e, int mode);
 int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out);
 int launch_field_out(Engine &e, double *out, double *ulj);
 int launch_skin(Engine &e);
 int launch_count_pairs(Engine &e, unsigned long long *d_out);
+int launch_remd_exchange(Engine &e, long long attempt);
+int launch_remd_velocities(Engine &e, long long attempt);
 int launch_set_rebuild(Engine &e);
 int launch_constrain_initial(Engine &e);
 int launch_start_md(Engine &e);

@@ -112,14 +112,14 @@ __global__ void __launch_bounds__(256) k_ks_field(const SysParams *__restrict__ 
   kres[4 * si + 3] = gz;
 }
 
-int launch_kspace(Engine &e, int want_virial) {
+int launch_kspace(Engine &e, int want_virial, hipStream_t st) {
   int maxk = 0, maxn = 0;
   for (auto &p : e.hsp)
     if (p.kmode == 0) { maxk = std::max(maxk, p.nk); maxn = std::max(maxn, p.nsite); }
   if (maxk == 0) return 0;
-  hipLaunchKernelGGL(k_ks_sf, dim3(maxk, e.nsys), dim3(256), 0, e.stream, e.d_sp,
+  hipLaunchKernelGGL(k_ks_sf, dim3(maxk, e.nsys), dim3(256), 0, st, e.d_sp,
                      (const double4 *)e.d_kvec, e.d_r, e.d_q, e.nsite, e.d_ksc);
-  hipLaunchKernelGGL(k_ks_field, dim3((maxn + 255) / 256, e.nsys), dim3(256), 0, e.stream,
+  hipLaunchKernelGGL(k_ks_field, dim3((maxn + 255) / 256, e.nsys), dim3(256), 0, st,
                      e.d_sp, (const double4 *)e.d_kvec, e.d_ksc, e.d_r, e.nsite, e.d_kres,
                      e.d_acc, want_virial);
   MM_HIP(hipGetLastError());

@@ -55,7 +55,7 @@ struct SysParams {
   long green0;                 // influence function offset (layout [ky][kz][kx])
   double T[9];                 // diag(n) L^-1, tensor_t layout (pppm.c:523-525)
   double RL[9];                // reciprocal lattice 2 pi (L^-1)^T, tensor_t layout
-  int bx, by, nbx, nby, bin0;  // spread column blocks
+  int bx, by, nbx, nby, bin0;  // spread column blocks (bx x by mesh columns each)
   // dynamics
   double dt;
   double bg_phi;               // neutralising background potential (or 0)
@@ -67,6 +67,24 @@ __host__ __device__ inline int mod_nonneg(int n, int m) {   // fns.h:48-58
   if (n >= 0) return n % m;
   int k = -n % m;
   return k > 0 ? m - k : 0;
+}
+
+// Mesh columns per spreading block along x and y: the smallest factor >= 2
+// of each mesh size (2 x 2 columns for even meshes: many small workgroups),
+// falling back to single columns where the block's points (bx by nz, 64-bit
+// each in LDS) would exceed maxpts or the block count would exceed maxbins.
+inline bool choose_spread_blocks(int nx, int ny, int nz, int maxpts, int maxbins, int &bx,
+                                 int &by) {
+  auto first_factor = [](int n) {
+    for (int b = 2; b <= 7; b++)
+      if (n % b == 0) return b;
+    return 1;
+  };
+  bx = first_factor(nx);
+  by = first_factor(ny);
+  if (bx * by * nz > maxpts) bx = 1;
+  if (bx * by * nz > maxpts) by = 1;
+  return bx * by * nz <= maxpts && (nx / bx) * (ny / by) <= maxbins;
 }
 
 __host__ __device__ inline void canonical_pair(int &i, int &j) {   // pbin.c:6

@@ -33,12 +33,14 @@ __device__ __forceinline__ double min_image_sq(double ax, double ay, double az, 
 __global__ void k_nl_assign(const SysParams *__restrict__ sp, const int *__restrict__ gfirst,
                             const double *__restrict__ r, int nsite_total,
                             const int *__restrict__ rebuild, int *__restrict__ gcell,
-                            double *__restrict__ last, int *__restrict__ cell_count) {
+                            double *__restrict__ last, int *__restrict__ cell_count,
+                            unsigned long long *__restrict__ builds) {
 #pragma clang fp contract(off)
   const int s = blockIdx.y;
   if (!rebuild[s]) return;
   const SysParams &P = sp[s];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) builds[s] += 1;
   if (i >= P.ngroup) return;
   const int g = P.group0 + i;
   const int site = gfirst[g];
@@ -230,7 +232,7 @@ int launch_nl_build(Engine &e, int force) {
   dim3 gg((e.maxg + tb - 1) / tb, e.nsys);
   const double *r = e.d_r;
   hipLaunchKernelGGL(k_nl_assign, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_gfirst, r, e.nsite,
-                     e.d_rebuild, e.d_gcell, e.d_last, e.d_cell_count);
+                     e.d_rebuild, e.d_gcell, e.d_last, e.d_cell_count, e.d_nlb);
   hipLaunchKernelGGL(k_nl_scan, dim3(1, e.nsys), dim3(1024), 0, e.stream, e.d_sp, e.d_rebuild,
                      e.d_cell_count, e.d_cell_start, e.d_cell_fill);
   hipLaunchKernelGGL(k_nl_scatter, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_rebuild, e.d_gcell,

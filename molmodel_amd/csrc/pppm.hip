@@ -22,28 +22,36 @@
 // polynomial tables tabulate (pppm.c:69-275), evaluated by recursion in FP32.
 #include "engine.hpp"
 #include "scan.hpp"
+#include "reduce.hpp"
 
 namespace mmamd {
 
 constexpr float FIX_SCALE = 4294967296.0f;   // 2^32
 constexpr double FIX_INV = 1.0 / 4294967296.0;
 
-__device__ __forceinline__ void bspline_w(float s, int order, float *w, float *ndw) {
-  // w[k] = M_p(u + p - 1 - k), u = s + 1/2; ndw[k] = -dw/ds
+template <int P>
+__device__ __forceinline__ void bspline_w(float s, float *w, float *ndw) {
+  // w[k] = M_P(u + P - 1 - k), u = s + 1/2; ndw[k] = -dw/ds (cardinal
+  // B-spline recursion, all loops unrolled for the compile-time order)
   const float u = s + 0.5f;
-  float m[MAXORDER + 1];
+  float m[P + 1];
 #pragma unroll
-  for (int k = 0; k <= MAXORDER; k++) m[k] = 0.f;
+  for (int k = 0; k <= P; k++) m[k] = 0.f;
   m[0] = 1.f;
-  for (int p = 2; p <= order; p++) {
-    if (p == order && ndw) {
-      for (int k = 0; k < order; k++) {
+#pragma unroll
+  for (int p = 2; p <= P; p++) {
+    if (p == P && ndw) {
+#pragma unroll
+      for (int k = 0; k < P; k++) {
         const float a = k < p - 1 ? m[k] : 0.f;
         const float b = k >= 1 ? m[k - 1] : 0.f;
-        ndw[order - 1 - k] = b - a;
+        ndw[P - 1 - k] = b - a;
       }
     }
+    constexpr float dummy = 0.f;
+    (void)dummy;
     const float inv = 1.f / (float)(p - 1);
+#pragma unroll
     for (int k = p - 1; k >= 0; k--) {
       const float x = u + (float)k;
       const float a = k < p - 1 ? m[k] : 0.f;
@@ -51,37 +59,55 @@ __device__ __forceinline__ void bspline_w(float s, int order, float *w, float *n
       m[k] = (x * a + ((float)p - x) * b) * inv;
     }
   }
-  if (order == 1 && ndw) ndw[0] = 0.f;
-  for (int k = 0; k < order; k++) w[order - 1 - k] = m[k];
+  if (P == 1 && ndw) ndw[0] = 0.f;
+#pragma unroll
+  for (int k = 0; k < P; k++) w[P - 1 - k] = m[k];
 }
 
-__global__ void k_p3m_grid(const SysParams *__restrict__ sp, const double *__restrict__ r,
-                           const double *__restrict__ q, int N, int4 *__restrict__ sgrid,
-                           float4 *__restrict__ sfrac, int *__restrict__ bin_count) {
+// wrap j + off into [0, n) for |off| < n (no integer division)
+__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+constexpr int MAXBINS = 4096;
+
+__global__ void __launch_bounds__(256) k_p3m_grid(const SysParams *__restrict__ sp,
+                                                  const double *__restrict__ r,
+                                                  const double *__restrict__ q, int N,
+                                                  int4 *__restrict__ sgrid,
+                                                  float4 *__restrict__ sfrac,
+                                                  int *__restrict__ bin_count) {
+  __shared__ int hist[MAXBINS];
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
   if (P.kmode != 1) return;
+  if ((int)(blockIdx.x * blockDim.x) >= P.nsite) return;
+  const int nb = P.nbx * P.nby;
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.nsite) return;
-  const int si = P.site0 + i;
-  const double x = r[si], y = r[N + si], z = r[2 * N + si];
-  const double *T = P.T;
-  double sx = T[0] * x + T[3] * y + T[6] * z;
-  double sy = T[1] * x + T[4] * y + T[7] * z;
-  double sz = T[2] * x + T[5] * y + T[8] * z;
-  int jx, jy, jz;
-  if (P.order & 1) {
-    jx = (int)floor(sx + 0.5); jy = (int)floor(sy + 0.5); jz = (int)floor(sz + 0.5);
-    sx -= jx; sy -= jy; sz -= jz;
-  } else {
-    jx = (int)floor(sx); jy = (int)floor(sy); jz = (int)floor(sz);
-    sx -= jx + 0.5; sy -= jy + 0.5; sz -= jz + 0.5;
+  if (i < P.nsite) {
+    const int si = P.site0 + i;
+    const double x = r[si], y = r[N + si], z = r[2 * N + si];
+    const double *T = P.T;
+    double sx = T[0] * x + T[3] * y + T[6] * z;
+    double sy = T[1] * x + T[4] * y + T[7] * z;
+    double sz = T[2] * x + T[5] * y + T[8] * z;
+    int jx, jy, jz;
+    if (P.order & 1) {
+      jx = (int)floor(sx + 0.5); jy = (int)floor(sy + 0.5); jz = (int)floor(sz + 0.5);
+      sx -= jx; sy -= jy; sz -= jz;
+    } else {
+      jx = (int)floor(sx); jy = (int)floor(sy); jz = (int)floor(sz);
+      sx -= jx + 0.5; sy -= jy + 0.5; sz -= jz + 0.5;
+    }
+    jx = mod_nonneg(jx, P.nx); jy = mod_nonneg(jy, P.ny); jz = mod_nonneg(jz, P.nz);
+    const int lb = (jx / P.bx) * P.nby + jy / P.by;
+    sgrid[si] = make_int4(jx, jy, jz, P.bin0 + lb);
+    sfrac[si] = make_float4((float)sx, (float)sy, (float)sz, (float)q[si]);
+    atomicAdd(&hist[lb], 1);
   }
-  jx = mod_nonneg(jx, P.nx); jy = mod_nonneg(jy, P.ny); jz = mod_nonneg(jz, P.nz);
-  const int bin = P.bin0 + (jx / P.bx) * P.nby + jy / P.by;
-  sgrid[si] = make_int4(jx, jy, jz, bin);
-  sfrac[si] = make_float4((float)sx, (float)sy, (float)sz, (float)q[si]);
-  atomicAdd(&bin_count[bin], 1);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nb; t += blockDim.x)
+    if (hist[t]) atomicAdd(&bin_count[P.bin0 + t], hist[t]);
 }
 
 __global__ void k_p3m_binscan(const SysParams *__restrict__ sp, int *__restrict__ bin_count,
@@ -93,74 +119,104 @@ __global__ void k_p3m_binscan(const SysParams *__restrict__ sp, int *__restrict_
                    P.site0);
 }
 
-__global__ void k_p3m_binscatter(const SysParams *__restrict__ sp, const int4 *__restrict__ sgrid,
-                                 const int *__restrict__ bin_start, int *__restrict__ bin_fill,
-                                 int *__restrict__ bin_sites) {
+__global__ void __launch_bounds__(256) k_p3m_binscatter(const SysParams *__restrict__ sp,
+                                                        const int4 *__restrict__ sgrid,
+                                                        const float4 *__restrict__ sfrac,
+                                                        const int *__restrict__ bin_start,
+                                                        int *__restrict__ bin_fill,
+                                                        int4 *__restrict__ bgrid,
+                                                        float4 *__restrict__ bfrac) {
+  __shared__ int cnt[MAXBINS];
+  __shared__ int base[MAXBINS];
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
   if (P.kmode != 1) return;
+  if ((int)(blockIdx.x * blockDim.x) >= P.nsite) return;
+  const int nb = P.nbx * P.nby;
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) cnt[t] = 0;
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.nsite) return;
-  const int si = P.site0 + i;
-  const int b = sgrid[si].w;
-  bin_sites[bin_start[b] + atomicAdd(&bin_fill[b], 1)] = si;
+  int lb = -1, rank = 0;
+  int4 g;
+  if (i < P.nsite) {
+    g = sgrid[P.site0 + i];
+    lb = g.w - P.bin0;
+    rank = atomicAdd(&cnt[lb], 1);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nb; t += blockDim.x)
+    if (cnt[t]) base[t] = bin_start[P.bin0 + t] + atomicAdd(&bin_fill[P.bin0 + t], cnt[t]);
+  __syncthreads();
+  if (lb >= 0) {
+    // copies in bin order, so the spreading blocks read them contiguously
+    const int pos = base[lb] + rank;
+    bgrid[pos] = make_int4(g.x, g.y, g.z, P.site0 + i);
+    bfrac[pos] = sfrac[P.site0 + i];
+  }
 }
 
 __device__ __forceinline__ int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
-constexpr int SPREAD_MAXPTS = 4 * 4 * 128;
+// LDS accumulator: bx * by columns of nz points (the host keeps the product
+// within SPREAD_MAXPTS, see choose_spread_blocks)
+constexpr int SPREAD_MAXPTS = 4096;
 
+template <int P>
 __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict__ sp,
-                                                    const int4 *__restrict__ sgrid,
-                                                    const float4 *__restrict__ sfrac,
                                                     const int *__restrict__ bin_start,
-                                                    const int *__restrict__ bin_sites,
+                                                    const int4 *__restrict__ bgrid,
+                                                    const float4 *__restrict__ bfrac,
                                                     float *__restrict__ mesh) {
   __shared__ unsigned long long acc[SPREAD_MAXPTS];
   const int s = blockIdx.y;
-  const SysParams &P = sp[s];
-  if (P.kmode != 1) return;
+  const SysParams &Q = sp[s];
+  if (Q.kmode != 1 || Q.order != P) return;
   const int blk = blockIdx.x;
-  if (blk >= P.nbx * P.nby) return;
-  const int bx = P.bx, by = P.by, nx = P.nx, ny = P.ny, nz = P.nz, p = P.order;
-  const int bi = blk / P.nby, bj = blk % P.nby;
+  if (blk >= Q.nbx * Q.nby) return;
+  const int bx = Q.bx, by = Q.by, nx = Q.nx, ny = Q.ny, nz = Q.nz;
+  const int bi = blk / Q.nby, bj = blk % Q.nby;
   const int X0 = bi * bx, Y0 = bj * by;
   const int npts = bx * by * nz;
   for (int t = threadIdx.x; t < npts; t += blockDim.x) acc[t] = 0ull;
   __syncthreads();
-  const int istart = -((p + 1) / 2 - 1);
+  constexpr int istart = -((P + 1) / 2 - 1);
   // neighbouring column blocks whose sites reach the owned columns
-  int xlo = floor_div(X0 - istart - p + 1, bx), xhi = floor_div(X0 + bx - 1 - istart, bx);
-  int ylo = floor_div(Y0 - istart - p + 1, by), yhi = floor_div(Y0 + by - 1 - istart, by);
-  if (xhi - xlo + 1 >= P.nbx) { xlo = 0; xhi = P.nbx - 1; }
-  if (yhi - ylo + 1 >= P.nby) { ylo = 0; yhi = P.nby - 1; }
+  int xlo = floor_div(X0 - istart - P + 1, bx), xhi = floor_div(X0 + bx - 1 - istart, bx);
+  int ylo = floor_div(Y0 - istart - P + 1, by), yhi = floor_div(Y0 + by - 1 - istart, by);
+  if (xhi - xlo + 1 >= Q.nbx) { xlo = 0; xhi = Q.nbx - 1; }
+  if (yhi - ylo + 1 >= Q.nby) { ylo = 0; yhi = Q.nby - 1; }
   for (int cx = xlo; cx <= xhi; cx++) {
-    const int ccx = mod_nonneg(cx, P.nbx);
+    const int ccx = mod_nonneg(cx, Q.nbx);
     for (int cy = ylo; cy <= yhi; cy++) {
-      const int ccy = mod_nonneg(cy, P.nby);
-      const int b = P.bin0 + ccx * P.nby + ccy;
+      const int ccy = mod_nonneg(cy, Q.nby);
+      const int b = Q.bin0 + ccx * Q.nby + ccy;
       const int b0 = bin_start[b], b1 = bin_start[b + 1];
       for (int t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
-        const int si = bin_sites[t];
-        const int4 g = sgrid[si];
-        const float4 f = sfrac[si];
-        float wx[MAXORDER], wy[MAXORDER], wz[MAXORDER];
-        bspline_w(f.x, p, wx, nullptr);
-        bspline_w(f.y, p, wy, nullptr);
-        bspline_w(f.z, p, wz, nullptr);
-        for (int a = 0; a < p; a++) {
-          const int gx = mod_nonneg(g.x + istart + a, nx) - X0;
-          if (gx < 0 || gx >= bx) continue;
-          for (int bb = 0; bb < p; bb++) {
-            const int gy = mod_nonneg(g.y + istart + bb, ny) - Y0;
-            if (gy < 0 || gy >= by) continue;
-            const float qw = f.w * wx[a] * wy[bb];
-            unsigned long long *col = acc + (gx * by + gy) * nz;
-            for (int c = 0; c < p; c++) {
-              const int gz = mod_nonneg(g.z + istart + c, nz);
-              const long long v = __float2ll_rn(qw * wz[c] * FIX_SCALE);
-              atomicAdd(col + gz, (unsigned long long)v);
-            }
+        const int4 g = bgrid[t];
+        const float4 f = bfrac[t];
+        float wx[P], wy[P], wz[P];
+        bspline_w<P>(f.x, wx, nullptr);
+        bspline_w<P>(f.y, wy, nullptr);
+        bspline_w<P>(f.z, wz, nullptr);
+        int lx[P], ly[P], iz[P];
+#pragma unroll
+        for (int a = 0; a < P; a++) {
+          lx[a] = wrap(g.x + istart + a, nx) - X0;
+          ly[a] = wrap(g.y + istart + a, ny) - Y0;
+          iz[a] = wrap(g.z + istart + a, nz);
+        }
+#pragma unroll
+        for (int a = 0; a < P; a++) {
+          if (lx[a] < 0 || lx[a] >= bx) continue;
+          const float qa = f.w * wx[a];
+#pragma unroll
+          for (int bb = 0; bb < P; bb++) {
+            if (ly[bb] < 0 || ly[bb] >= by) continue;
+            const float qw = qa * wy[bb];
+            unsigned long long *col = acc + (lx[a] * by + ly[bb]) * nz;
+#pragma unroll
+            for (int c = 0; c < P; c++)
+              atomicAdd(col + iz[c], (unsigned long long)__float2ll_rn(qw * wz[c] * FIX_SCALE));
           }
         }
       }
@@ -169,15 +225,15 @@ __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict_
   __syncthreads();
   for (int t = threadIdx.x; t < npts; t += blockDim.x) {
     const int gz = t % nz, gy = (t / nz) % by, gx = t / (nz * by);
-    mesh[P.mesh0 + ((long)(X0 + gx) * ny + (Y0 + gy)) * nz + gz] =
+    mesh[Q.mesh0 + ((long)(X0 + gx) * ny + (Y0 + gy)) * nz + gz] =
         (float)((double)(long long)acc[t] * FIX_INV);
   }
 }
 
 // ---------------- line FFTs in LDS ----------------
-constexpr int FFT_TB = 256;
 constexpr int FFT_T = 16;                 // threads per line
-constexpr int FFT_LINES = FFT_TB / FFT_T; // lines per block
+constexpr int FFT_LINES = P3M_FFT_LINES;  // lines per block
+constexpr int FFT_TB = FFT_T * FFT_LINES; // one wave
 constexpr int FFT_MAXN = 128;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -186,65 +242,68 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-__device__ __forceinline__ int fft_radices(int n, int *rad) {
-  int k = 0;
-  while (n % 4 == 0) { rad[k++] = 4; n /= 4; }
-  while (n % 2 == 0) { rad[k++] = 2; n /= 2; }
-  while (n % 3 == 0) { rad[k++] = 3; n /= 3; }
-  while (n % 5 == 0) { rad[k++] = 5; n /= 5; }
-  while (n % 7 == 0) { rad[k++] = 7; n /= 7; }
-  return k;
-}
-
 // twiddle exp(sign*2*pi*i*t/n) from tw[t] = exp(-2 pi i t/n)
 __device__ __forceinline__ float2 twid(const float2 *tw, int t, int n, int sign) {
   const float2 w = tw[t % n];
   return sign < 0 ? w : make_float2(w.x, -w.y);
 }
 
-// Stockham autosort FFT of FFT_LINES lines of length n held in x (line l at
-// x + l*FFT_MAXN); every thread of the block must call it.  Returns the
-// buffer holding the result.
-__device__ float2 *lds_fft_lines(float2 *x, float2 *y, int n, const float2 *tw, int sign) {
-  int rad[16];
-  const int nr = fft_radices(n, rad);
-  const int l = threadIdx.x / FFT_T, tid = threadIdx.x % FFT_T;
-  int p = 1;
-  for (int st = 0; st < nr; st++) {
-    const int R = rad[st];
-    const int nb = n / R;
-    const int tstep = n / (p * R);
-    float2 *xl = x + l * FFT_MAXN, *yl = y + l * FFT_MAXN;
-    for (int i = tid; i < nb; i += FFT_T) {
-      const int k = i % p;
-      float2 u[7], v[7];
-      for (int r = 0; r < R; r++) {
-        u[r] = xl[i + r * nb];
-        if (r > 0 && k > 0) u[r] = cmul(u[r], twid(tw, r * k * tstep, n, sign));
-      }
-      if (R == 2) {
-        v[0] = cadd(u[0], u[1]); v[1] = csub(u[0], u[1]);
-      } else if (R == 4) {
-        const float2 a = cadd(u[0], u[2]), b = csub(u[0], u[2]);
-        const float2 c = cadd(u[1], u[3]), d = csub(u[1], u[3]);
-        // multiply d by sign*i
-        const float2 di = sign < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
-        v[0] = cadd(a, c); v[2] = csub(a, c);
-        v[1] = cadd(b, di); v[3] = csub(b, di);
-      } else {
-        const int ws = n / R;
-        for (int r = 0; r < R; r++) {
-          float2 acc = u[0];
-          for (int t = 1; t < R; t++) acc = cadd(acc, cmul(u[t], twid(tw, ((r * t) % R) * ws, n, sign)));
-          v[r] = acc;
-        }
-      }
-      const int j = (i - k) * R + k;
-      for (int r = 0; r < R; r++) yl[j + r * p] = v[r];
+// one Stockham stage of radix R over a line of length n (stride p)
+template <int R>
+__device__ __forceinline__ void fft_stage(const float2 *xl, float2 *yl, int n, int p,
+                                          const float2 *tw, int sign, int tid) {
+  const int nb = n / R;
+  const int tstep = n / (p * R);
+  for (int i = tid; i < nb; i += FFT_T) {
+    const int k = i % p;
+    float2 u[R], v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      u[r] = xl[i + r * nb];
+      if (r > 0 && k > 0) u[r] = cmul(u[r], twid(tw, r * k * tstep, n, sign));
     }
+    if (R == 2) {
+      v[0] = cadd(u[0], u[1]); v[1] = csub(u[0], u[1]);
+    } else if (R == 4) {
+      const float2 a = cadd(u[0], u[2]), b = csub(u[0], u[2]);
+      const float2 c = cadd(u[1], u[3]), d = csub(u[1], u[3]);
+      const float2 di = sign < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+      v[0] = cadd(a, c); v[2] = csub(a, c);
+      v[1] = cadd(b, di); v[3] = csub(b, di);
+    } else {
+      const int ws = n / R;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        float2 acc = u[0];
+#pragma unroll
+        for (int t = 1; t < R; t++) acc = cadd(acc, cmul(u[t], twid(tw, ((r * t) % R) * ws, n, sign)));
+        v[r] = acc;
+      }
+    }
+    const int j = (i - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) yl[j + r * p] = v[r];
+  }
+}
+
+// Stockham autosort FFT of FFT_LINES lines of length n held in x (line l at
+// x + l*FFT_MAXN); every thread of the block must call it.  Radices are
+// taken greedily (4, 2, 3, 5, 7).  Returns the buffer holding the result.
+__device__ float2 *lds_fft_lines(float2 *x, float2 *y, int n, const float2 *tw, int sign) {
+  const int l = threadIdx.x / FFT_T, tid = threadIdx.x % FFT_T;
+  int p = 1, rem = n;
+  while (rem > 1) {
+    float2 *xl = x + l * FFT_MAXN, *yl = y + l * FFT_MAXN;
+    int R;
+    if (rem % 4 == 0) { R = 4; fft_stage<4>(xl, yl, n, p, tw, sign, tid); }
+    else if (rem % 2 == 0) { R = 2; fft_stage<2>(xl, yl, n, p, tw, sign, tid); }
+    else if (rem % 3 == 0) { R = 3; fft_stage<3>(xl, yl, n, p, tw, sign, tid); }
+    else if (rem % 5 == 0) { R = 5; fft_stage<5>(xl, yl, n, p, tw, sign, tid); }
+    else { R = 7; fft_stage<7>(xl, yl, n, p, tw, sign, tid); }
     __syncthreads();
     float2 *t = x; x = y; y = t;
     p *= R;
+    rem /= R;
   }
   return x;
 }
@@ -332,6 +391,7 @@ __global__ void __launch_bounds__(FFT_TB) k_fft_xgreen(const SysParams *__restri
   __shared__ float2 stw[FFT_MAXN];
   __shared__ double red[FFT_TB / 64][10];
   __shared__ int last;
+  (void)acc;
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
   if (P.kmode != 1) return;
@@ -382,87 +442,71 @@ __global__ void __launch_bounds__(FFT_TB) k_fft_xgreen(const SysParams *__restri
     float2 *rl = res + l * FFT_MAXN;
     for (int t = tid; t < nx; t += FFT_T) src[t * xstride] = rl[t];
   }
-  // deterministic block partials, summed in block order by the last block
-  for (int o = 32; o >= 1; o >>= 1) {
-    e += __shfl_xor(e, o);
-    for (int c = 0; c < 9; c++) w[c] += __shfl_xor(w[c], o);
-  }
-  const int wid = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[wid][0] = e;
-    for (int c = 0; c < 9; c++) red[wid][1 + c] = w[c];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int ww = 0; ww < FFT_TB / 64; ww++)
-      for (int c = 0; c < 10; c++) t[c] += red[ww][c];
-    double *pp = part + ((long)s * maxblk + blockIdx.x) * 10;
-    for (int c = 0; c < 10; c++) pp[c] = t[c];
-    __threadfence();
-    last = (atomicAdd(&done[s], 1) == nblk - 1);
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    double t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < nblk; b++) {
-      const volatile double *pp = part + ((long)s * maxblk + b) * 10;
-      for (int c = 0; c < 10; c++) t[c] += pp[c];
-    }
-    acc[s * NACC + ACC_KSPACE_E] = t[0];
-    for (int c = 0; c < 9; c++) kvir[9 * s + c] = want_virial ? t[1 + c] : 0.0;
-    done[s] = 0;
+  // deterministic block partials, reduced by the last block
+  double vals[10] = {e, w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
+  if (finish_partials<10, 10, FFT_TB>(vals, part + (long)s * maxblk * 10, nblk, blockIdx.x,
+                                      &done[s], red, &last)) {
+    acc[s * NACC + ACC_KSPACE_E] = vals[0];
+    for (int c = 0; c < 9; c++) kvir[9 * s + c] = want_virial ? vals[1 + c] : 0.0;
   }
 }
 
-__global__ void k_p3m_gather(const SysParams *__restrict__ sp, const int4 *__restrict__ sgrid,
-                             const float4 *__restrict__ sfrac, const float *__restrict__ mesh,
-                             double *__restrict__ kres) {
+template <int P>
+__global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict__ sp,
+                                                    const int4 *__restrict__ sgrid,
+                                                    const float4 *__restrict__ sfrac,
+                                                    const float *__restrict__ mesh,
+                                                    double *__restrict__ kres) {
   const int s = blockIdx.y;
-  const SysParams &P = sp[s];
-  if (P.kmode != 1) return;
+  const SysParams &Q = sp[s];
+  if (Q.kmode != 1 || Q.order != P) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.nsite) return;
-  const int si = P.site0 + i;
+  if (i >= Q.nsite) return;
+  const int si = Q.site0 + i;
   const int4 g = sgrid[si];
   const float4 f = sfrac[si];
-  const int p = P.order, nx = P.nx, ny = P.ny, nz = P.nz;
-  const int istart = -((p + 1) / 2 - 1);
-  float wx[MAXORDER], wy[MAXORDER], wz[MAXORDER], dx[MAXORDER], dy[MAXORDER], dz[MAXORDER];
-  bspline_w(f.x, p, wx, dx);
-  bspline_w(f.y, p, wy, dy);
-  bspline_w(f.z, p, wz, dz);
-  int iz[MAXORDER];
-  for (int c = 0; c < p; c++) iz[c] = mod_nonneg(g.z + istart + c, nz);
+  const int nx = Q.nx, ny = Q.ny, nz = Q.nz;
+  constexpr int istart = -((P + 1) / 2 - 1);
+  float wx[P], wy[P], wz[P], dx[P], dy[P], dz[P];
+  bspline_w<P>(f.x, wx, dx);
+  bspline_w<P>(f.y, wy, dy);
+  bspline_w<P>(f.z, wz, dz);
+  int ix[P], iy[P], iz[P];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    ix[a] = wrap(g.x + istart + a, nx);
+    iy[a] = wrap(g.y + istart + a, ny);
+    iz[a] = wrap(g.z + istart + a, nz);
+  }
+  const float *m0 = mesh + Q.mesh0;
   float ph = 0.f, e0 = 0.f, e1 = 0.f, e2 = 0.f;
-  for (int a = 0; a < p; a++) {
-    const int ix = mod_nonneg(g.x + istart + a, nx);
-    for (int b = 0; b < p; b++) {
-      const int iy = mod_nonneg(g.y + istart + b, ny);
-      const float *col = mesh + P.mesh0 + ((long)ix * ny + iy) * nz;
-      const float wxy = wx[a] * wy[b];
-      const float dxy = dx[a] * wy[b], wdy = wx[a] * dy[b];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+#pragma unroll
+    for (int b = 0; b < P; b++) {
+      const float *col = m0 + ((long)ix[a] * ny + iy[b]) * nz;
       float cp = 0.f, cz = 0.f;
-      for (int c = 0; c < p; c++) {
+#pragma unroll
+      for (int c = 0; c < P; c++) {
         const float pm = col[iz[c]];
         cp += wz[c] * pm;
         cz += dz[c] * pm;
       }
+      const float wxy = wx[a] * wy[b];
       ph += wxy * cp;
-      e0 += dxy * cp;
-      e1 += wdy * cp;
+      e0 += dx[a] * wy[b] * cp;
+      e1 += wx[a] * dy[b] * cp;
       e2 += wxy * cz;
     }
   }
-  const double *T = P.T;
+  const double *T = Q.T;
   kres[4 * si] = (double)ph;
   kres[4 * si + 1] = (double)e0 * T[0] + (double)e1 * T[1] + (double)e2 * T[2];
   kres[4 * si + 2] = (double)e0 * T[3] + (double)e1 * T[4] + (double)e2 * T[5];
   kres[4 * si + 3] = (double)e0 * T[6] + (double)e1 * T[7] + (double)e2 * T[8];
 }
 
-int launch_p3m(Engine &e, int want_virial) {
+int launch_p3m(Engine &e, int want_virial, hipStream_t st) {
   int maxn = 0, maxbins = 0, maxz = 0, maxy = 0, maxx = 0;
   for (auto &p : e.hsp)
     if (p.kmode == 1) {
@@ -476,27 +520,38 @@ int launch_p3m(Engine &e, int want_virial) {
   if (maxn == 0) return 0;
   const int tb = 256;
   dim3 gs((maxn + tb - 1) / tb, e.nsys);
-  hipLaunchKernelGGL(k_p3m_grid, gs, dim3(tb), 0, e.stream, e.d_sp, e.d_r, e.d_q, e.nsite,
+  hipLaunchKernelGGL(k_p3m_grid, gs, dim3(tb), 0, st, e.d_sp, e.d_r, e.d_q, e.nsite,
                      e.d_sgrid, e.d_sfrac, e.d_bin_count);
-  hipLaunchKernelGGL(k_p3m_binscan, dim3(1, e.nsys), dim3(1024), 0, e.stream, e.d_sp,
+  hipLaunchKernelGGL(k_p3m_binscan, dim3(1, e.nsys), dim3(1024), 0, st, e.d_sp,
                      e.d_bin_count, e.d_bin_start, e.d_bin_fill);
-  hipLaunchKernelGGL(k_p3m_binscatter, gs, dim3(tb), 0, e.stream, e.d_sp, e.d_sgrid,
-                     e.d_bin_start, e.d_bin_fill, e.d_bin_sites);
-  hipLaunchKernelGGL(k_p3m_spread, dim3(maxbins, e.nsys), dim3(256), 0, e.stream, e.d_sp,
-                     e.d_sgrid, e.d_sfrac, e.d_bin_start, e.d_bin_sites, e.d_mesh);
-  hipLaunchKernelGGL(k_fft_pass, dim3(maxz, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp, e.d_twid,
+  hipLaunchKernelGGL(k_p3m_binscatter, gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid,
+                     e.d_sfrac, e.d_bin_start, e.d_bin_fill, e.d_bgrid, e.d_bfrac);
+  bool used[MAXORDER + 1] = {false};
+  for (auto &p : e.hsp)
+    if (p.kmode == 1) used[p.order] = true;
+#define SPREAD(PO)                                                                          \
+  if (used[PO])                                                                             \
+    hipLaunchKernelGGL(k_p3m_spread<PO>, dim3(maxbins, e.nsys), dim3(256), 0, st,     \
+                       e.d_sp, e.d_bin_start, e.d_bgrid, e.d_bfrac, e.d_mesh);
+  SPREAD(1) SPREAD(2) SPREAD(3) SPREAD(4) SPREAD(5) SPREAD(6) SPREAD(7)
+#undef SPREAD
+  hipLaunchKernelGGL(k_fft_pass, dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
                      e.d_mesh, e.d_cmesh, 0);
-  hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp, e.d_twid,
+  hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
                      e.d_mesh, e.d_cmesh, 1);
-  hipLaunchKernelGGL(k_fft_xgreen, dim3(maxx, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp,
+  hipLaunchKernelGGL(k_fft_xgreen, dim3(maxx, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
                      e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
                      e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
-  hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp, e.d_twid,
+  hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
                      e.d_mesh, e.d_cmesh, 3);
-  hipLaunchKernelGGL(k_fft_pass, dim3(maxz, e.nsys), dim3(FFT_TB), 0, e.stream, e.d_sp, e.d_twid,
+  hipLaunchKernelGGL(k_fft_pass, dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
                      e.d_mesh, e.d_cmesh, 4);
-  hipLaunchKernelGGL(k_p3m_gather, gs, dim3(tb), 0, e.stream, e.d_sp, e.d_sgrid, e.d_sfrac,
-                     e.d_mesh, e.d_kres);
+#define GATHER(PO)                                                                        \
+  if (used[PO])                                                                           \
+    hipLaunchKernelGGL(k_p3m_gather<PO>, gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid,     \
+                       e.d_sfrac, e.d_mesh, e.d_kres);
+  GATHER(1) GATHER(2) GATHER(3) GATHER(4) GATHER(5) GATHER(6) GATHER(7)
+#undef GATHER
   MM_HIP(hipGetLastError());
   return 0;
 }

@@ -473,11 +473,11 @@ extern "C" p3m_t p3m_new(int nsite, const double ptv[9], double grid_spacing, in
   P.nx = p->ngrid[0]; P.ny = p->ngrid[1]; P.nz = p->ngrid[2];
   P.order = assignment_order;
   const int nzh = P.nz / 2 + 1;
-  for (int b = 4; b >= 1; b--) if (P.nx % b == 0) { P.bx = b; break; }
-  for (int b = 4; b >= 1; b--) if (P.ny % b == 0) { P.by = b; break; }
+  if (!choose_spread_blocks(P.nx, P.ny, P.nz, 4096, 4096, P.bx, P.by))
+    die_msg("p3m_new", "mesh dimensions give no valid spreading column blocks");
   P.nbx = P.nx / P.bx;
   P.nby = P.ny / P.by;
-  e.nblk_part = std::max(1, (P.ny * nzh + 15) / 16);
+  e.nblk_part = std::max(1, (P.ny * nzh + P3M_FFT_LINES - 1) / P3M_FFT_LINES);
   const size_t M = (size_t)P.nx * P.ny * P.nz, Mc = (size_t)P.nx * P.ny * nzh;
   HIPCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
   HIPCHK(hipMalloc((void **)&e.d_sp, sizeof(SysParams)));
@@ -497,7 +497,8 @@ extern "C" p3m_t p3m_new(int nsite, const double ptv[9], double grid_spacing, in
   HIPCHK(hipMalloc((void **)&e.d_bin_fill, sizeof(int) * nb));
   HIPCHK(hipMemset(e.d_bin_count, 0, sizeof(int) * nb));
   HIPCHK(hipMemset(e.d_bin_fill, 0, sizeof(int) * nb));
-  HIPCHK(hipMalloc((void **)&e.d_bin_sites, sizeof(int) * N));
+  HIPCHK(hipMalloc((void **)&e.d_bgrid, sizeof(int4) * N));
+  HIPCHK(hipMalloc((void **)&e.d_bfrac, sizeof(float4) * N));
   HIPCHK(hipMalloc((void **)&e.d_part, sizeof(double) * 16 * e.nblk_part));
   HIPCHK(hipMalloc((void **)&e.d_done, sizeof(int) * 4));
   HIPCHK(hipMemset(e.d_done, 0, sizeof(int) * 4));
@@ -582,7 +583,7 @@ extern "C" void p3m_delete(p3m_t p) {
   if (!p) return;
   Engine &e = p->e;
   void *ptrs[] = {e.d_sp, e.d_r, e.d_q, e.d_kres, e.d_acc, e.d_mesh, e.d_cmesh, e.d_green,
-                  e.d_sgrid, e.d_sfrac, e.d_bin_count, e.d_bin_start, e.d_bin_fill, e.d_bin_sites,
+                  e.d_sgrid, e.d_sfrac, e.d_bin_count, e.d_bin_start, e.d_bin_fill, e.d_bgrid, e.d_bfrac,
                   e.d_part, e.d_done, e.d_twid};
   for (void *q : ptrs)
     if (q) hipFree(q);

@@ -12,11 +12,12 @@
 //   * the switching derivative times the sum of unswitched pair energies acts
 //     on the first sites (rspace.c:479-489).
 // MI355X mapping: groups are cell-sorted; 16 lanes serve one i-group and
-// stride over its full neighbour row (both directions, so every force lands
-// on the lane's own i-group and no atomics are needed); the i-group lives
-// in registers; site-pair arithmetic is FP32; each group pair's FP32 partial
-// sums are added into FP64 accumulators; the 16 lanes of an i-group are
-// reduced with cross-lane shuffles at the end.
+// walk its full neighbour row (both directions, so every force lands on the
+// lane's own i-group and no atomics are needed); the i-group lives in
+// registers; neighbours inside the cutoff are compacted into an LDS queue
+// before the site-pair arithmetic (FP32) runs; each group pair's FP32
+// partial sums are added into FP64 accumulators; the 16 lanes of an i-group
+// are reduced with cross-lane shuffles at the end.
 #include "engine.hpp"
 
 namespace mmamd {
@@ -58,6 +59,20 @@ constexpr int GPB = PAIR_TB / LANES;   // i-groups per block
 constexpr int MAXTAB = 1536;
 constexpr int MAXLJ = 64;
 
+constexpr int QCAP = 32;      // queue entries per i-group (one batch of 16 + a fill of 16)
+
+// The pair kernel.  Each 16-lane segment owns one i-group and alternates
+// two phases over its neighbour row:
+//   filter:  16 candidates at a time; FP64 minimum image and the
+//            r^2 > rc^2 / switching decisions of rspace.c:423-445; accepted
+//            neighbours are compacted into a per-segment LDS queue with a
+//            ballot (FP32 shift, switching value and derivative);
+//   compute: once every segment of the wavefront holds a full batch (or has
+//            no candidates left), each lane takes one queued group pair and
+//            evaluates its site pairs in FP32, adding FP32 per-pair sums
+//            into FP64 accumulators.
+// Lanes therefore spend the expensive phase almost only on pairs inside the
+// cutoff.
 template <bool FIELD, bool VIRIAL>
 __global__ void __launch_bounds__(PAIR_TB) k_pair(
     const SysParams *__restrict__ sp, const double4 *__restrict__ gpos,
@@ -67,6 +82,8 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
     double *__restrict__ pf, double *__restrict__ plj, double *__restrict__ gacc) {
   __shared__ float4 stab[MAXTAB];
   __shared__ float2 slj[MAXLJ];
+  __shared__ float4 qd[GPB][QCAP];    // Dx, Dy, Dz, switching value
+  __shared__ float2 qs[GPB][QCAP];    // switching derivative, neighbour (int bits)
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
   if ((int)(blockIdx.x * GPB) >= P.ngroup) return;
@@ -76,7 +93,9 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
   __syncthreads();
 
   const int lane = threadIdx.x & (LANES - 1);
-  const int i = blockIdx.x * GPB + (threadIdx.x / LANES);
+  const int segw = (threadIdx.x & 63) / LANES;     // segment within the wavefront
+  const int seg = threadIdx.x / LANES;             // segment within the block
+  const int i = blockIdx.x * GPB + seg;
   const bool active = i < P.ngroup;
   const int gs = P.group0 + (active ? i : 0);
 
@@ -92,7 +111,6 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
     mki[k] = k < nsi ? (b & 3) : 0;
     tyi[k] = b >> 2;
   }
-  // accumulators
   double ax[NSI], ay[NSI], az[NSI], ap[NSI];
   double lx[NSI], ly[NSI], lz[NSI];
   double ulj = 0;
@@ -105,112 +123,159 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
   const double L = P.L, invL = P.invL, rhi2 = P.rhi2, rlo2 = P.rlo2;
   const float tAmin = P.tabAmin, tMid = P.tabMid, tAinv = P.tabAinv, tBinv = P.tabBinv,
               tBmax = P.tabBmax;
-  const int ntA = P.ntabA, nty = P.ntype;
+  // Both table segments in one index space: x_A(r2) = (r2 - Amin) / dxA and
+  // x_B(r2) = nA + (r2 - Mid) / dxB meet at r2 = Mid, and A is the steeper
+  // map, so the table index is min(x_A, x_B) and the segment is the one
+  // giving the minimum.
+  const float offA = -tAmin * tAinv, offB = (float)P.ntabA - tMid * tBinv;
+  const float m2A = -2.f * tAinv, m2B = -2.f * tBinv;
+  const int nty = P.ntype;
   const float eta = (float)P.eta;
   const int cnt = active ? nnb[gs] : 0;
   const int *row = nbr + (long)gs * nbcap;
+  float4 *myqd = qd[seg];
+  float2 *myqs = qs[seg];
+  int qn = 0, t0 = 0;
 
-  for (int t = lane; t < cnt; t += LANES) {
-    const int hs = row[t];
-    const double4 pj = gpos[hs];
-    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-    dx -= L * floor(dx * invL + 0.5);
-    dy -= L * floor(dy * invL + 0.5);
-    dz -= L * floor(dz * invL + 0.5);
-    const double r2g = dx * dx + dy * dy + dz * dz;
-    if (r2g > rhi2) continue;
-    double sm = 1.0, dsm = 0.0;
-    if (r2g > rlo2) {                       // rspace.c:52-58
-      const double a = P.inv_w2;
-      const double z = (r2g - rlo2) * a;
-      sm = ((-6 * z + 15) * z - 10) * z * z * z + 1;
-      dsm = ((60 * z - 120) * z + 60) * z * z * a;
+  while (true) {
+    // ---- filter until every segment holds a batch or has no candidates
+    while (__any(qn < LANES && t0 < cnt)) {
+      const bool fill = qn < LANES && t0 < cnt;
+      bool take = false;
+      float4 ent = make_float4(0.f, 0.f, 0.f, 1.f);
+      float dsmf = 0.f;
+      int hs = 0;
+      if (fill && t0 + lane < cnt) {
+        hs = row[t0 + lane];
+        const double4 pj = gpos[hs];
+        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        dx -= L * floor(dx * invL + 0.5);
+        dy -= L * floor(dy * invL + 0.5);
+        dz -= L * floor(dz * invL + 0.5);
+        const double r2g = dx * dx + dy * dy + dz * dz;
+        take = !(r2g > rhi2);
+        double sm = 1.0, dsm = 0.0;
+        if (r2g > rlo2) {                       // rspace.c:52-58
+          const double a = P.inv_w2;
+          const double z = (r2g - rlo2) * a;
+          sm = ((-6 * z + 15) * z - 10) * z * z * z + 1;
+          dsm = ((60 * z - 120) * z + 60) * z * z * a;
+        }
+        ent = make_float4((float)dx, (float)dy, (float)dz, (float)sm);
+        dsmf = (float)dsm;
+      }
+      const unsigned long long bal = __ballot(take);
+      const unsigned sb = (unsigned)(bal >> (segw * LANES)) & 0xffffu;
+      if (take) {
+        const int pos = qn + __popc(sb & ((1u << lane) - 1u));
+        myqd[pos] = ent;
+        myqs[pos] = make_float2(dsmf, __int_as_float(hs));
+      }
+      if (fill) {
+        qn += __popc(sb);
+        t0 += LANES;
+      }
+      __builtin_amdgcn_wave_barrier();
     }
-    const float smf = (float)sm;
-    const int2 mj = gmeta[hs];
-    const int nsj = mj.x;
-    const float Dx = (float)dx, Dy = (float)dy, Dz = (float)dz;
-    float fx[NSI], fy[NSI], fz[NSI], ph[NSI], gx[NSI], gy[NSI], gz[NSI];
+    if (!__any(qn > 0)) break;
+    // ---- compute one batch per segment
+    const int nb = qn < LANES ? qn : LANES;
+    if (lane < nb) {
+      const float4 ent = myqd[qn - nb + lane];
+      const float2 e2 = myqs[qn - nb + lane];
+      const int hs = __float_as_int(e2.y);
+      const float smf = ent.w, dsmf = e2.x;
+      const int2 mj = gmeta[hs];
+      const int nsj = mj.x;
+      const float Dx = ent.x, Dy = ent.y, Dz = ent.z;
+      float fx[NSI], fy[NSI], fz[NSI], ph[NSI], gx[NSI], gy[NSI], gz[NSI];
 #pragma unroll
-    for (int k = 0; k < NSI; k++) { fx[k] = fy[k] = fz[k] = ph[k] = 0.f; gx[k] = gy[k] = gz[k] = 0.f; }
-    float tmps = 0.f, uu = 0.f;
+      for (int k = 0; k < NSI; k++) { fx[k] = fy[k] = fz[k] = ph[k] = 0.f; gx[k] = gy[k] = gz[k] = 0.f; }
+      float tmps = 0.f, uu = 0.f;
 #pragma unroll
-    for (int l = 0; l < NSI; l++) {
-      if (l >= nsj) break;
-      const float4 oj = spk[(long)hs * MAXS + l];
-      const int bj = (mj.y >> (8 * l)) & 0xff;
-      const int mkj = bj & 3, tyj = bj >> 2;
-      const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
-      const float qj = oj.w;
+      for (int l = 0; l < NSI; l++) {
+        if (l >= nsj) break;
+        const float4 oj = spk[(long)hs * MAXS + l];
+        const int bj = (mj.y >> (8 * l)) & 0xff;
+        const int mkj = bj & 3, tyj = bj >> 2;
+        const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
+        const float qj = oj.w;
+        const float qs = qj * smf;        // smoothed charge of site l
+        const float qsA = qs * m2A, qsB = qs * m2B;
+        float tq = 0.f;                   // sum_k q_k b0_k (unsmoothed energy / q_l)
+#pragma unroll
+        for (int k = 0; k < NSI; k++) {
+          const int m = mki[k] & mkj;
+          if (!m) continue;
+          const float rx = px + oi[k].x, ry = py + oi[k].y, rz = pz + oi[k].z;
+          const float r2 = rx * rx + ry * ry + rz * rz;
+          if (m & SITE_HAS_Q) {
+            float b0, e;
+            if (r2 >= tAmin && r2 < tBmax) {
+              const float xa = fmaf(r2, tAinv, offA), xb = fmaf(r2, tBinv, offB);
+              const bool inA = xa < xb;
+              const float xx = inA ? xa : xb;
+              const int ii = (int)xx;
+              const float fr = xx - (float)ii;
+              const float4 c = stab[ii];
+              const float p2 = fmaf(c.w, fr, c.z), p1 = fmaf(p2, fr, c.y);
+              b0 = fmaf(p1, fr, c.x);
+              // b1 = -2/dx * d(cubic)/dt (the derivative from the Horner
+              // intermediates), times the smoothed charge
+              e = fmaf(fr, fmaf(c.w, fr, p2), p1) * (inA ? qsA : qsB);
+            } else {                            // rspace.c:120-125, outside the table
+              const float rm = sqrtf(r2), er = eta * rm;
+              b0 = erfcf(er) / rm;
+              e = qs * (b0 + eta * 1.1283791670955126f * expf(-er * er)) / r2;
+            }
+            tq += oi[k].w * b0;
+            ph[k] += qs * b0;
+            fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
+          }
+          if (m & SITE_HAS_LJ) {
+            const float2 c = slj[tyi[k] * nty + tyj];
+            const float ir2 = __builtin_amdgcn_rcpf(r2);
+            const float ir6 = ir2 * ir2 * ir2;
+            const float bi6 = c.x * ir6, ai12 = c.y * ir6 * ir6;
+            const float u = ai12 - bi6;
+            const float du = (12.f * ai12 - 6.f * bi6) * ir2 * smf;
+            uu += u;
+            gx[k] += du * rx; gy[k] += du * ry; gz[k] += du * rz;
+          }
+        }
+        tmps += qj * tq;
+      }
+      tmps += uu;
+      uu *= smf;
+      // smoothing force on the first site (rspace.c:479-489)
+      const float fs = dsmf * tmps;
+      const float sx = fs * Dx, sy = fs * Dy, sz = fs * Dz;
+      double Fx = sx, Fy = sy, Fz = sz;
 #pragma unroll
       for (int k = 0; k < NSI; k++) {
-        const int m = mki[k] & mkj;
-        if (!m) continue;
-        const float rx = px + oi[k].x, ry = py + oi[k].y, rz = pz + oi[k].z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (m & SITE_HAS_Q) {
-          float b0, b1;
-          if (r2 >= tAmin && r2 < tBmax) {
-            const bool hi = r2 >= tMid;
-            const float inv = hi ? tBinv : tAinv;
-            const float xx = (r2 - (hi ? tMid : tAmin)) * inv;
-            const int ii = (int)xx;
-            const float fr = xx - (float)ii;
-            const float4 c = stab[(hi ? ntA : 0) + ii];
-            b0 = ((c.w * fr + c.z) * fr + c.y) * fr + c.x;
-            b1 = -2.f * inv * ((3.f * c.w * fr + 2.f * c.z) * fr + c.y);
-          } else {                            // rspace.c:120-125, exact
-            const double rm = sqrt((double)r2), er = (double)eta * rm;
-            const double e0 = erfc(er) / rm;
-            b0 = (float)e0;
-            b1 = (float)((e0 + P.eta * 1.1283791670955126 * exp(-er * er)) / (double)r2);
-          }
-          const float qiqj = oi[k].w * qj;
-          tmps += qiqj * b0;
-          const float b0s = b0 * smf, b1s = b1 * smf;
-          ph[k] += qj * b0s;
-          const float e = qj * b1s;
-          fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
-        }
-        if (m & SITE_HAS_LJ) {
-          const float2 c = slj[tyi[k] * nty + tyj];
-          const float ir2 = 1.f / r2;
-          const float ir6 = ir2 * ir2 * ir2;
-          const float bi6 = c.x * ir6, ai12 = c.y * ir6 * ir6;
-          const float u = ai12 - bi6;
-          const float du = (12.f * ai12 - 6.f * bi6) * ir2 * smf;
-          tmps += u;
-          uu += u * smf;
-          gx[k] += du * rx; gy[k] += du * ry; gz[k] += du * rz;
+        ap[k] += (double)ph[k];
+        if (FIELD) {
+          ax[k] += (double)fx[k]; ay[k] += (double)fy[k]; az[k] += (double)fz[k];
+          lx[k] += (double)gx[k]; ly[k] += (double)gy[k]; lz[k] += (double)gz[k];
+        } else {
+          const float qk = oi[k].w;
+          const float tx = qk * fx[k] + gx[k], ty = qk * fy[k] + gy[k], tz = qk * fz[k] + gz[k];
+          ax[k] += (double)tx; ay[k] += (double)ty; az[k] += (double)tz;
+          if (VIRIAL) { Fx += tx; Fy += ty; Fz += tz; }
         }
       }
-    }
-    // smoothing force on the first site (rspace.c:479-489)
-    const double fs = dsm * (double)tmps;
-    const double sx = fs * dx, sy = fs * dy, sz = fs * dz;
-    double Fx = sx, Fy = sy, Fz = sz;
-#pragma unroll
-    for (int k = 0; k < NSI; k++) {
-      ap[k] += (double)ph[k];
-      if (FIELD) {
-        ax[k] += (double)fx[k]; ay[k] += (double)fy[k]; az[k] += (double)fz[k];
-        lx[k] += (double)gx[k]; ly[k] += (double)gy[k]; lz[k] += (double)gz[k];
-      } else {
-        const float qk = oi[k].w;
-        const double tx = (double)(qk * fx[k] + gx[k]), ty = (double)(qk * fy[k] + gy[k]),
-                     tz = (double)(qk * fz[k] + gz[k]);
-        ax[k] += tx; ay[k] += ty; az[k] += tz;
-        if (VIRIAL) { Fx += tx; Fy += ty; Fz += tz; }
+      if (FIELD) { lx[0] += (double)sx; ly[0] += (double)sy; lz[0] += (double)sz; }
+      else { ax[0] += (double)sx; ay[0] += (double)sy; az[0] += (double)sz; }
+      ulj += (double)uu;
+      if (VIRIAL) {   // 1/2 D (x) F_gh per directed pair
+        const double dx = Dx, dy = Dy, dz = Dz;
+        w[0] += 0.5 * dx * Fx; w[1] += 0.5 * dx * Fy; w[2] += 0.5 * dx * Fz;
+        w[3] += 0.5 * dy * Fx; w[4] += 0.5 * dy * Fy; w[5] += 0.5 * dy * Fz;
+        w[6] += 0.5 * dz * Fx; w[7] += 0.5 * dz * Fy; w[8] += 0.5 * dz * Fz;
       }
     }
-    if (FIELD) { lx[0] += sx; ly[0] += sy; lz[0] += sz; }
-    else { ax[0] += sx; ay[0] += sy; az[0] += sz; }
-    ulj += (double)uu;
-    if (VIRIAL) {   // 1/2 D (x) F_gh per directed pair
-      w[0] += 0.5 * dx * Fx; w[1] += 0.5 * dx * Fy; w[2] += 0.5 * dx * Fz;
-      w[3] += 0.5 * dy * Fx; w[4] += 0.5 * dy * Fy; w[5] += 0.5 * dy * Fz;
-      w[6] += 0.5 * dz * Fx; w[7] += 0.5 * dz * Fy; w[8] += 0.5 * dz * Fz;
-    }
+    qn -= nb;
+    __builtin_amdgcn_wave_barrier();
   }
   // reduce the 16 lanes of each i-group
 #pragma unroll

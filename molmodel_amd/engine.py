@@ -61,6 +61,13 @@ def lib():
         L.mm_stream.argtypes = [_P]
         L.mm_sync.argtypes = [_P]
         L.mm_engine_sizes.argtypes = [_P, _P, _P]
+        L.mm_remd_setup.argtypes = [_P, _I, _P, _P, C.c_ulonglong]
+        L.mm_remd_run.argtypes = [_P, _I, _I, _I, _I]
+        L.mm_remd_stats.argtypes = [_P, _P, _P]
+        L.mm_tune_kspace.argtypes = [_D, _I, _D, _D, _I, _P]
+        L.mm_param_gradients.argtypes = [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
+        L.mm_nl_builds.restype = C.c_longlong
+        L.mm_nl_builds.argtypes = [_P]
         _lib = L
     return _lib
 
@@ -71,6 +78,28 @@ def _ptr(a):
 
 class MMError(RuntimeError):
     pass
+
+
+class KspaceChoice(C.Structure):
+    _fields_ = [("kmode", _I), ("eta", _D), ("kc", _D), ("mesh", _I * 3), ("order", _I),
+                ("err_rspace", _D), ("err_kspace", _D), ("cost", _D)]
+
+
+def tune_kspace(boxl, nsite, rc, tol=1e-4, allow_ewald=True):
+    """The library's reciprocal-space tuner (host only, no GPU needed).
+    Returns a parameter dict for a waterbox-style params entry plus the
+    estimates: {eta, kmode, kc | mesh, order, err_rspace, err_kspace, cost}."""
+    ch = KspaceChoice()
+    _check(lib().mm_tune_kspace(float(boxl), int(nsite), float(rc), float(tol),
+                                1 if allow_ewald else 0, C.byref(ch)))
+    out = dict(eta=ch.eta, kmode=ch.kmode, err_rspace=ch.err_rspace,
+               err_kspace=ch.err_kspace, cost=ch.cost)
+    if ch.kmode == 0:
+        out["kc"] = ch.kc
+    else:
+        out["mesh"] = tuple(ch.mesh)
+        out["order"] = ch.order
+    return out
 
 
 def _check(rc):
@@ -210,3 +239,50 @@ class Engine:
 
     def sync(self):
         _check(lib().mm_sync(self.h))
+
+    def nl_builds(self):
+        return _check(lib().mm_nl_builds(self.h))
+
+    # parameter gradients -----------------------------------------------------
+    def param_gradients(self, sys=0, f_ref=None, maxtypes=16):
+        """dE/d(q, sigma, epsilon) and dL/d(...) per site type of system
+        `sys` for L = sum |F - f_ref|^2; see mm_param_gradients."""
+        n = self.systems[sys].nsite
+        nt = np.zeros(1, dtype=np.int32)
+        st = np.zeros(n, dtype=np.int32)
+        tp = np.zeros(3 * maxtypes)
+        dE = np.zeros(3 * maxtypes)
+        dL = np.zeros(3 * maxtypes)
+        loss = np.zeros(1)
+        berr = np.zeros(1)
+        fr = None if f_ref is None else np.ascontiguousarray(f_ref, dtype=np.float64).reshape(-1)
+        _check(lib().mm_param_gradients(self.h, sys, _ptr(fr), maxtypes, _ptr(nt), _ptr(st),
+                                        _ptr(tp), _ptr(dE), _ptr(dL), _ptr(loss), _ptr(berr)))
+        k = int(nt[0])
+        return dict(ntype=k, site_type=st, params=tp[:3 * k].reshape(k, 3),
+                    dE=dE[:3 * k].reshape(k, 3), dL=dL[:3 * k].reshape(k, 3),
+                    loss=float(loss[0]), basis_err=float(berr[0]))
+
+    # replica exchange --------------------------------------------------------
+    def remd_setup(self, ladders, kT, seed=1):
+        """ladders: lists of system indices by ascending temperature;
+        kT: per system, kcal/mol."""
+        flat = []
+        for i, lad in enumerate(ladders):
+            if i:
+                flat.append(-1)
+            flat.extend(int(x) for x in lad)
+        lad = np.asarray(flat, dtype=np.int32)
+        kT = np.ascontiguousarray(kT, dtype=np.float64)
+        assert kT.size == self.nsys
+        _check(lib().mm_remd_setup(self.h, lad.size, _ptr(lad), _ptr(kT), int(seed)))
+
+    def remd_run(self, nintervals, steps, reinit_velocities=True, graph=True):
+        _check(lib().mm_remd_run(self.h, int(nintervals), int(steps),
+                                 1 if reinit_velocities else 0, 1 if graph else 0))
+
+    def remd_stats(self):
+        st = np.zeros((self.nsys, 2), dtype=np.int64)
+        cfg = np.zeros(self.nsys, dtype=np.int32)
+        _check(lib().mm_remd_stats(self.h, _ptr(st), _ptr(cfg)))
+        return st, cfg

@@ -268,3 +268,51 @@ def default_skin(rc, boxl):
     if rc + nlw >= 0.5 * boxl:
         nlw = 0.499 * boxl - rc
     return nlw
+
+
+def smooth_mesh_size(n: int) -> int:
+    """Smallest size >= n (and >= 8) whose prime factors are 2, 3, 5, 7."""
+    n = max(int(n), 8)
+    while True:
+        m = n
+        for f in (2, 3, 5, 7):
+            while m % f == 0:
+                m //= f
+        if m == 1:
+            return n
+        n += 1
+
+
+def remd_batch(nrep: int = 16, t_lo: float = 300.0, t_hi: float = 450.0, seed: int = 7,
+               sizes=((6, 18.64), (8, 24.86), (10, 31.07)), kspace=None):
+    """Replica-exchange batch: for each (n, L) in sizes, nrep replicas of an
+    n^3-water box at temperatures geometric from t_lo to t_hi.  Replicas of
+    one size share the box and topology and start from different seeded
+    configurations.  kspace(boxl, nsite, rc) -> dict of reciprocal-space
+    parameters (eta, kmode, kc or mesh/order) chooses them per size; the
+    default is Ewald (eta 0.35, kc 7) below 20 A and P3M (eta 0.32, order 5,
+    mesh spacing about 1 A) above.  Returns (systems, params, kT, ladders)
+    with kT in kcal/mol per system and one ladder (ascending T) per size."""
+    systems, params, kT, ladders = [], [], [], []
+    temps = [t_lo * (t_hi / t_lo) ** (k / max(nrep - 1, 1)) for k in range(nrep)]
+    for si, (n, L) in enumerate(sizes):
+        lad = []
+        rc = 9.0 if L < 20 else min(10.0, 0.49 * L)
+        if kspace is not None:
+            kp = dict(kspace(L, 3 * n ** 3, rc))
+        elif L < 20:
+            kp = dict(eta=0.35, kmode=0, kc=7.0)
+        else:
+            m = smooth_mesh_size(round(L))
+            kp = dict(eta=0.32, kmode=1, mesh=(m, m, m), order=5)
+        for k, T in enumerate(temps):
+            s = water_box(n, L, seed + 1000 * si + k, temperature=T)
+            p = dict(rc=rc, smoothing_width=1.0, **kp)
+            p.setdefault("skin", default_skin(p["rc"], L))
+            p.update(nalias=2, ik=0, include_lj_correction=1)
+            lad.append(len(systems))
+            systems.append(s)
+            params.append(p)
+            kT.append(K_to_kcal_mol(T))
+        ladders.append(lad)
+    return systems, params, kT, ladders

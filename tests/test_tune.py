@@ -1,0 +1,57 @@
+"""The library's reciprocal-space tuner (host code, no GPU).
+
+Pins:
+  * the tuner's P3M error estimate equals the oracle's chi (the restatement
+    of p3m_init's estimate, pppm.c:313-393) for the chosen mesh, eta, order;
+  * that chi tracks the measured RMS force error of oracle P3M against a
+    converged oracle Ewald sum within a factor of 10 over three decades of
+    mesh/order choices (a dropped or misplaced term in the estimate would
+    break this scaling);
+  * every choice meets its own estimates, and tightening the tolerance never
+    makes the modelled cost cheaper.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from molmodel_amd import waterbox as W
+from molmodel_amd.engine import tune_kspace
+
+SIZES = [(18.64, 648, 9.0), (24.86, 1536, 10.0), (31.07, 3000, 10.0)]
+
+
+@pytest.mark.parametrize("boxl,nsite,rc", SIZES)
+def test_choice_meets_estimates(boxl, nsite, rc):
+    costs = []
+    for tol in (1e-3, 1e-4):
+        c = tune_kspace(boxl, nsite, rc, tol)
+        target = tol / np.sqrt(2)
+        assert c["err_rspace"] <= target * (1 + 1e-9)
+        # P3M is searched with the 27-term alias sum and reported with 125
+        assert c["err_kspace"] <= target * 1.05
+        assert 0.1 < c["eta"] * rc < 5
+        if c["kmode"] == 1:
+            n = c["mesh"][0]
+            assert W.smooth_mesh_size(n) == n
+            _, chi = O.green(c["mesh"], boxl, c["eta"], c["order"], nalias=2)
+            # same estimate; the tuner factorises the alias sum, so its
+            # cancelling differences are summed in another order
+            assert abs(chi - c["err_kspace"]) <= 1e-5 * chi
+        costs.append(c["cost"])
+    assert costs[1] >= costs[0]
+
+
+def test_chi_tracks_measured_p3m_error():
+    s, _ = W.config("water216_small")
+    eta = 0.35
+    _, ev0, _, _ = O.kspace_parts(s, 14, eta)
+    f0 = s.q[:, None] * ev0
+    ratios, chis = [], []
+    for order, n in ((3, 8), (3, 16), (5, 8), (5, 12), (5, 24)):
+        _, ev, _, _ = O.p3m_parts(s, dict(mesh=(n, n, n), eta=eta, order=order))
+        rms = np.sqrt(np.mean(np.sum((s.q[:, None] * ev - f0) ** 2, 1)))
+        _, chi = O.green((n, n, n), s.boxl, eta, order)
+        ratios.append(rms / chi)
+        chis.append(chi)
+    assert max(chis) / min(chis) > 300
+    assert 1.0 < min(ratios) and max(ratios) < 10.0, ratios
