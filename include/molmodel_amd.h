@@ -132,14 +132,28 @@ long long mm_nl_builds(mm_engine *e);   /* neighbour-list builds so far, all sys
 int mm_md_setup(mm_engine *e, double dt, double bond, double angle_deg, double tol);
 int mm_md_run(mm_engine *e, int nsteps, int use_graph);
 
+/* NeighborList (nbrlist.hpp:8-40) on the device for n points in a cubic box
+ * of side boxl: NeighborList(n, cutoff, width) + set_boundary_conditions
+ * (nbrlist.cpp:9-17, 84-123).  update = coordinates_changed(r)
+ * (nbrlist.cpp:51-77): r is AoS 3n; returns 1 if the list was rebuilt, 0 if
+ * the skin test kept it.  rows: the pair set in PairBinary layout (each pair
+ * in the row of its pbin.c:6 owner), row_start[n+1] offsets into cols, at
+ * most cap columns written; returns the pair count.  The same set as the
+ * reference's for the same input; order within a row is the device's. */
+typedef struct mm_nlist mm_nlist;
+int mm_nlist_create(int n, double cutoff, double width, double boxl, mm_nlist **out);
+int mm_nlist_update(mm_nlist *nl, const double *r);
+long mm_nlist_rows(mm_nlist *nl, int *row_start, int *cols, long cap);
+int mm_nlist_destroy(mm_nlist *nl);
+
 /* Replica exchange among the engine's systems (mdyn.cpp:317-351, 476-497).
  * ladder: system indices by ascending temperature, ladders separated by -1
  * (systems on one ladder must be copies of one system: same sites, groups,
  * box and k-space); kT: nsys values in kcal/mol.  Requires mm_md_setup.
- * run: nintervals x (steps MD steps (0: none), then, if reinit_velocities, new
- * Maxwell-Boltzmann velocities at each system's kT, then one exchange
- * attempt per neighbouring pair from the hottest down, positions swapped,
- * forces recomputed).  stats: tries and accepts per system (as the lower
+ * run: nintervals x (steps MD steps (0: none), then one exchange attempt
+ * per neighbouring pair from the hottest down with positions swapped,
+ * then, if reinit_velocities, new Maxwell-Boltzmann velocities at each
+ * system's kT, then forces recomputed).  stats: tries and accepts per system (as the lower
  * member of a pair), 2 per system; config: which starting configuration
  * each system now holds. */
 int mm_remd_setup(mm_engine *e, int nladder, const int *ladder, const double *kT,

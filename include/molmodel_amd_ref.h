@@ -74,6 +74,33 @@ void RSpaceEwald(int nsite, int ngroup, const int (*group)[2], const cart_t *r,
                  const struct PairBinary *one_four,
                  double elec_1_4_scale_factor,
                  double lj_1_4_scale_factor);
+/* rspace.h:52-62: phi only, same pair sums; the excluded-pair correction
+ * without the minimum image, as rspace.c:747-751 does. */
+void RSpaceEwaldPhi(int nsite, int ngroup, const int (*group)[2], const cart_t *r,
+                    enum bcondtype_t bctype, const tensor_t *latv,
+                    int *const *neighbors, const int *number_of_neighbors,
+                    const struct PairBinary *group_excluded,
+                    const struct PairBinary *excluded,
+                    double rlo2, double rhi2,
+                    const int *site_mask, const double *q,
+                    double *phi,
+                    double eta, const struct Chebyshev *rspace_chebyshev,
+                    const double *rscreen,
+                    const struct PairBinary *one_four,
+                    double elec_1_4_scale_factor);
+
+/* rspace.h:64-72: potential at ngrid points from every site with a mask
+ * bit (the real-space part of ElecLJ::electrostatic_potential_at_gridpoints,
+ * eleclj.cpp:1295-1341; the caller adds kspace_phi over points + sites). */
+void RSpaceEwaldPhiAtGridpoints(int nsite, int ngrid, const cart_t *rsite,
+                                const cart_t *rgrid, enum bcondtype_t bctype,
+                                const tensor_t *latv,
+                                double rlo2, double rhi2,
+                                const int *site_mask,
+                                const double *q,
+                                double *phi,
+                                double eta, const struct Chebyshev *rspace_chebyshev,
+                                const double *rscreen);
 
 /* ---- kspace.h:10-18 ---- */
 struct kspace;

@@ -377,6 +377,102 @@ long mm_nl_pairs(mm_engine *m, int sys, int *pairs, long maxp) {
   return np;
 }
 
+// ------------------------------------------------ standalone neighbour list
+// NeighborList (nbrlist.hpp:8-40) for n points in a cubic box: a
+// single-system engine whose groups are the points themselves.
+struct mm_nlist {
+  mm_engine *m = nullptr;
+  int n = 0;
+  long long builds = 0;
+};
+
+int mm_nlist_create(int n, double cutoff, double width, double boxl, mm_nlist **out) {
+  if (!out || n <= 0 || !(cutoff > 0) || !(width >= 0) || !(boxl > 0)) {
+    set_last_error("mm_nlist_create: bad arguments");
+    return MM_ERR_ARG;
+  }
+  *out = nullptr;
+  std::vector<int> group(2 * (size_t)n), mask(n, 0);
+  std::vector<double> zero(n, 0.0);
+  for (int i = 0; i < n; i++) { group[2 * i] = i; group[2 * i + 1] = i + 1; }
+  mm_system_desc d;
+  memset(&d, 0, sizeof d);
+  d.nsite = n; d.ngroup = n; d.group = group.data();
+  d.q = zero.data(); d.sigma = zero.data(); d.epsilon = zero.data(); d.site_mask = mask.data();
+  d.boxl = boxl; d.rc = cutoff; d.smoothing_width = 0; d.skin = width;
+  d.eta = 3.0 / cutoff; d.kmode = 2;
+  mm_engine *m = nullptr;
+  TRY(mm_engine_create(1, &d, &m));
+  mm_nlist *nl = new mm_nlist();
+  nl->m = m;
+  nl->n = n;
+  *out = nl;
+  return MM_OK;
+}
+
+int mm_nlist_destroy(mm_nlist *nl) {
+  if (!nl) return MM_OK;
+  mm_engine_destroy(nl->m);
+  delete nl;
+  return MM_OK;
+}
+
+// coordinates_changed (nbrlist.cpp:51-77): skin test against the positions
+// of the last build, rebuild when it fires (always on the first call)
+int mm_nlist_update(mm_nlist *nl, const double *r) {
+  if (!nl || !r) { set_last_error("mm_nlist_update: bad arguments"); return MM_ERR_ARG; }
+  mm_engine *m = nl->m;
+  Engine &e = m->e;
+  TRY(put_aos(m, e.d_r, r));
+  for (int attempt = 0; attempt < 4; attempt++) {
+    if (e.first_eval) {
+      TRY(launch_nl_build(e, 1));
+      e.first_eval = 0;
+    } else {
+      TRY(launch_set_rebuild(e, 0));   // the skin test alone decides
+      TRY(launch_skin(e));
+      TRY(launch_nl_build(e, 0));
+    }
+    m->pos_dirty = 0;
+    int f = 0;
+    TRY(read_err(e, &f));
+    if (f & ERR_NL_OVERFLOW) {
+      TRY(clear_err(e));
+      TRY(grow_nbr(e));
+      continue;
+    }
+    int rebuilt = 0;
+    MM_HIP(hipMemcpy(&rebuilt, e.d_rebuild, sizeof(int), hipMemcpyDeviceToHost));
+    nl->builds += rebuilt ? 1 : 0;
+    return rebuilt ? 1 : 0;
+  }
+  set_last_error("neighbour list overflow persisted");
+  return MM_ERR_DEVICE;
+}
+
+// the pair set as PairBinary rows (pbin.c:6 owner rule): row_start[n+1]
+// offsets into cols; returns the pair count (cols written up to cap)
+long mm_nlist_rows(mm_nlist *nl, int *row_start, int *cols, long cap) {
+  if (!nl) { set_last_error("mm_nlist_rows: bad arguments"); return MM_ERR_ARG; }
+  const long np = mm_nl_pairs(nl->m, 0, nullptr, 0);
+  if (np < 0) return np;
+  std::vector<int> pairs(2 * (size_t)np + 2);
+  mm_nl_pairs(nl->m, 0, pairs.data(), np);
+  std::vector<int> cnt(nl->n + 1, 0);
+  for (long k = 0; k < np; k++) cnt[pairs[2 * k] + 1]++;
+  for (int i = 0; i < nl->n; i++) cnt[i + 1] += cnt[i];
+  if (row_start)
+    for (int i = 0; i <= nl->n; i++) row_start[i] = cnt[i];
+  if (cols) {
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (long k = 0; k < np; k++) {
+      const long at = fill[pairs[2 * k]]++;
+      if (at < cap) cols[at] = pairs[2 * k + 1];
+    }
+  }
+  return np;
+}
+
 int mm_md_setup(mm_engine *m, double dt, double bond, double angle_deg, double tol) {
   CHECK_ENGINE(m);
   Engine &e = m->e;
@@ -514,10 +610,12 @@ int mm_remd_run(mm_engine *m, int nintervals, int steps, int reinit_velocities, 
   for (int it = 0; it < nintervals; it++) {
     if (steps > 0) TRY(md_run_async(m, steps, use_graph));
     const long long a = e.remd_attempt++;
-    // end of the interval's last step: velocity redraw, then the exchange
-    // attempt (mdyn.cpp:163-167, 185-186)
-    if (reinit_velocities) TRY(launch_remd_velocities(e, a));
+    // end of the interval's last step: the exchange attempt (mdyn.cpp:185-186),
+    // then the velocity redraw (mdyn.cpp:163-167; the reference redraws
+    // before exchanging, which leaves swapped rigid molecules with velocities
+    // constrained for the configuration they left)
     TRY(launch_remd_exchange(e, a));
+    if (reinit_velocities) TRY(launch_remd_velocities(e, a));
     // forces and energies of the configurations now held; swapped systems
     // rebuild their neighbour lists first
     TRY(enqueue_forces(m, 0, 0));

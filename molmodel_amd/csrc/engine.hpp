@@ -164,12 +164,14 @@ inline int launch_kspace(Engine &e, int want_virial) { return launch_kspace(e, w
 inline int launch_p3m(Engine &e, int want_virial) { return launch_p3m(e, want_virial, e.stream); }
 int launch_finish(Engine &e, int mode);
 int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out);
-int launch_field_out(Engine &e, double *out, double *ulj);
+int launch_field_out(Engine &e, double *out, double *ulj, int min_image = 1);
 int launch_skin(Engine &e);
 int launch_count_pairs(Engine &e, unsigned long long *d_out);
 int launch_remd_exchange(Engine &e, long long attempt);
+int launch_phi_points(Engine &e, const double4 *sites, int nsite, const double *pts, int npts,
+                      double *phi);
 int launch_remd_velocities(Engine &e, long long attempt);
-int launch_set_rebuild(Engine &e);
+int launch_set_rebuild(Engine &e, int value = 1);
 int launch_constrain_initial(Engine &e);
 int launch_start_md(Engine &e);
 

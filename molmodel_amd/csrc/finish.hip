@@ -25,9 +25,13 @@
 namespace mmamd {
 
 constexpr int FIN_TB = 64;
-constexpr int NPART = 14;   // elec, lj, kin, vir[9], top1, top2
+// reduced per system: elec, lj, kin, [vir[9]], top1, top2
+template <bool VIR> struct FinSlots {
+  static constexpr int N = VIR ? 14 : 5;
+  static constexpr int TOP = N - 2;
+};
 
-template <int MODE>
+template <int MODE, bool VIR>
 __global__ void __launch_bounds__(FIN_TB) k_finish(
     const SysParams *__restrict__ sp, const int *__restrict__ order, const int *__restrict__ gfirst,
     const int *__restrict__ gns, const int *__restrict__ gexcl, const double *__restrict__ q,
@@ -37,6 +41,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
     int N, int want_virial, double bond, double angle_rad, double tol,
     double *__restrict__ part, int *__restrict__ done, int maxblk, double *__restrict__ acc,
     int *__restrict__ rebuild, int *__restrict__ err) {
+  constexpr int NPART = FinSlots<VIR>::N, TOP = FinSlots<VIR>::TOP;
   __shared__ double red[FIN_TB / 64][NPART];
   __shared__ int islast;
   const int s = blockIdx.y;
@@ -92,7 +97,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
           const double fq = qq[k] * qq[l] * b1;
           F[k][0] += fq * d0; F[k][1] += fq * d1; F[k][2] += fq * d2;
           F[l][0] -= fq * d0; F[l][1] -= fq * d1; F[l][2] -= fq * d2;
-          if (want_virial) {
+          if constexpr (VIR) {
             vals[3] += d0 * fq * d0; vals[4] += d0 * fq * d1; vals[5] += d0 * fq * d2;
             vals[6] += d1 * fq * d0; vals[7] += d1 * fq * d1; vals[8] += d1 * fq * d2;
             vals[9] += d2 * fq * d0; vals[10] += d2 * fq * d1; vals[11] += d2 * fq * d2;
@@ -103,9 +108,10 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
     for (int k = 0; k < K; k++) vals[0] += 0.5 * qq[k] * ph[k];
     const double *ga = gacc + (long)gs * 10;
     vals[1] = ga[0];
-    if (want_virial)
+    if constexpr (VIR) {
 #pragma unroll
       for (int c = 0; c < 9; c++) vals[3 + c] += ga[1 + c];
+    }
 #pragma unroll
     for (int k = 0; k < K; k++)
       if (k < ns) {
@@ -166,18 +172,18 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
     // nbrlist.cpp:64-73 on the first site
     const double dx = x[0][0] - last[3 * g], dy = x[0][1] - last[3 * g + 1],
                  dz = x[0][2] - last[3 * g + 2];
-    vals[12] = sqrt(dx * dx + dy * dy + dz * dz);
+    vals[TOP] = sqrt(dx * dx + dy * dy + dz * dz);
   }
-  if (finish_partials<NPART, 12, FIN_TB>(vals, part + (long)s * maxblk * NPART, nblk, blockIdx.x,
+  if (finish_partials<NPART, TOP, FIN_TB>(vals, part + (long)s * maxblk * NPART, nblk, blockIdx.x,
                                          &done[s], red, &islast)) {
     double *a = acc + s * NACC;
     a[ACC_ELEC] = vals[0];
     a[ACC_LJ] = vals[1];
     if (MODE & FIN_KICK) a[ACC_KIN] = vals[2];
-    for (int c = 0; c < 9; c++) a[ACC_VIR + c] = vals[3 + c];
-    a[ACC_TOP1] = vals[12];
-    a[ACC_TOP2] = vals[13];
-    rebuild[s] = (vals[12] + vals[13] > P.skin) ? 1 : 0;   // nbrlist.cpp:75-76
+    for (int c = 0; c < 9; c++) a[ACC_VIR + c] = VIR ? vals[3 + c] : 0.0;
+    a[ACC_TOP1] = vals[TOP];
+    a[ACC_TOP2] = vals[TOP + 1];
+    rebuild[s] = (vals[TOP] + vals[TOP + 1] > P.skin) ? 1 : 0;   // nbrlist.cpp:75-76
     if (!(vals[0] == vals[0]) || !(vals[1] == vals[1])) atomicOr(err, ERR_NAN);
   }
 }
@@ -212,7 +218,7 @@ __global__ void k_field_out(const SysParams *__restrict__ sp, const int *__restr
                             const int *__restrict__ gexcl, const double *__restrict__ q,
                             const double *__restrict__ pf, const double *__restrict__ plj,
                             const double *__restrict__ gacc, const double *__restrict__ r, int N,
-                            double *__restrict__ out, double *__restrict__ ulj) {
+                            double *__restrict__ out, double *__restrict__ ulj, int min_image) {
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -231,9 +237,11 @@ __global__ void k_field_out(const SysParams *__restrict__ sp, const int *__restr
     for (int l = k + 1; l < ns; l++) {
       if (!((ex >> (k * MAXS + l)) & 1)) continue;
       double d0 = x[k][0] - x[l][0], d1 = x[k][1] - x[l][1], d2 = x[k][2] - x[l][2];
-      d0 -= P.L * floor(d0 * P.invL + 0.5);
-      d1 -= P.L * floor(d1 * P.invL + 0.5);
-      d2 -= P.L * floor(d2 * P.invL + 0.5);
+      if (min_image) {   // RSpaceEwald does (rspace.c:501-505), RSpaceEwaldPhi not (:747-751)
+        d0 -= P.L * floor(d0 * P.invL + 0.5);
+        d1 -= P.L * floor(d1 * P.invL + 0.5);
+        d2 -= P.L * floor(d2 * P.invL + 0.5);
+      }
       const double r2 = d0 * d0 + d1 * d1 + d2 * d2, rm = sqrt(r2), er = P.eta * rm;
       const double b0 = -erf(er) / rm;
       const double b1 = (b0 + P.eta * 1.1283791670955126 * exp(-er * er)) / r2;
@@ -252,11 +260,11 @@ __global__ void k_field_out(const SysParams *__restrict__ sp, const int *__restr
   atomicAdd(&ulj[s], gacc[(long)gs * 10]);
 }
 
-int launch_field_out(Engine &e, double *out, double *ulj) {
+int launch_field_out(Engine &e, double *out, double *ulj, int min_image) {
   const int tb = 256;
   hipLaunchKernelGGL(k_field_out, dim3((e.maxg + tb - 1) / tb, e.nsys), dim3(tb), 0, e.stream,
                      e.d_sp, e.d_order, e.d_gfirst, e.d_gns, e.d_gexcl, e.d_q, e.d_pf, e.d_plj,
-                     e.d_gacc, e.d_r, e.nsite, out, ulj);
+                     e.d_gacc, e.d_r, e.nsite, out, ulj, min_image);
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -267,12 +275,18 @@ int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out) {
     e.d_gacc, e.d_kres, e.d_r, e.d_v, e.d_f, phi_out, e.d_last, e.nsite, want_virial, e.bond,  \
     e.angle * (M_PI / 180.0), e.tol, e.d_part, e.d_done, e.nblk_part, e.d_acc, e.d_rebuild,   \
     e.d_err
+#define FIN_CASE(M)                                                                            \
+  if (want_virial)                                                                             \
+    hipLaunchKernelGGL((k_finish<M, true>), grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS);         \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_finish<M, false>), grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS);
   switch (mode) {
-    case 0: hipLaunchKernelGGL(k_finish<0>, grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS); break;
-    case 1: hipLaunchKernelGGL(k_finish<1>, grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS); break;
-    case 2: hipLaunchKernelGGL(k_finish<2>, grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS); break;
-    default: hipLaunchKernelGGL(k_finish<3>, grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS); break;
+    case 0: FIN_CASE(0) break;
+    case 1: FIN_CASE(1) break;
+    case 2: FIN_CASE(2) break;
+    default: FIN_CASE(3) break;
   }
+#undef FIN_CASE
 #undef FIN_ARGS
   MM_HIP(hipGetLastError());
   return 0;

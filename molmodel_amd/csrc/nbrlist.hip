@@ -216,9 +216,9 @@ int launch_skin(Engine &e) {
   return 0;
 }
 
-int launch_set_rebuild(Engine &e) {
+int launch_set_rebuild(Engine &e, int value) {
   hipLaunchKernelGGL(k_set_flags, dim3((e.nsys + 63) / 64), dim3(64), 0, e.stream, e.d_rebuild,
-                     e.nsys, 1);
+                     e.nsys, value);
   MM_HIP(hipGetLastError());
   return 0;
 }

@@ -378,6 +378,7 @@ __global__ void __launch_bounds__(FFT_TB) k_fft_pass(const SysParams *__restrict
 // x lines: forward FFT, multiply by the influence function, inverse FFT.
 // Also the mesh energy 1/2 sum G |rho_k|^2 and its virial
 // W_ab = sum e_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 eta^2))).
+template <bool VIR>
 __global__ void __launch_bounds__(FFT_TB) k_fft_xgreen(const SysParams *__restrict__ sp,
                                                        const float2 *__restrict__ twid_all,
                                                        const float *__restrict__ green,
@@ -389,7 +390,8 @@ __global__ void __launch_bounds__(FFT_TB) k_fft_xgreen(const SysParams *__restri
                                                        int maxblk) {
   __shared__ float2 bx_[FFT_LINES * FFT_MAXN], by_[FFT_LINES * FFT_MAXN];
   __shared__ float2 stw[FFT_MAXN];
-  __shared__ double red[FFT_TB / 64][10];
+  constexpr int NV = VIR ? 10 : 1;   // energy [, virial]
+  __shared__ double red[FFT_TB / 64][NV];
   __shared__ int last;
   (void)acc;
   const int s = blockIdx.y;
@@ -424,7 +426,7 @@ __global__ void __launch_bounds__(FFT_TB) k_fft_xgreen(const SysParams *__restri
       const float gg = g[t];
       const double ek = 0.5 * mult * (double)gg * ((double)v.x * v.x + (double)v.y * v.y);
       e += ek;
-      if (want_virial && gg != 0.f) {
+      if (VIR && gg != 0.f) {
         const int jx = t > nx / 2 ? t - nx : t;
         double k[3];
         for (int a = 0; a < 3; a++) k[a] = P.RL[a] * jx + P.RL[3 + a] * jy + P.RL[6 + a] * kz;
@@ -443,11 +445,14 @@ __global__ void __launch_bounds__(FFT_TB) k_fft_xgreen(const SysParams *__restri
     for (int t = tid; t < nx; t += FFT_T) src[t * xstride] = rl[t];
   }
   // deterministic block partials, reduced by the last block
-  double vals[10] = {e, w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
-  if (finish_partials<10, 10, FFT_TB>(vals, part + (long)s * maxblk * 10, nblk, blockIdx.x,
+  double vals[NV];
+  vals[0] = e;
+  if constexpr (VIR)
+    for (int c = 0; c < 9; c++) vals[1 + c] = w[c];
+  if (finish_partials<NV, NV, FFT_TB>(vals, part + (long)s * maxblk * NV, nblk, blockIdx.x,
                                       &done[s], red, &last)) {
     acc[s * NACC + ACC_KSPACE_E] = vals[0];
-    for (int c = 0; c < 9; c++) kvir[9 * s + c] = want_virial ? vals[1 + c] : 0.0;
+    for (int c = 0; c < 9; c++) kvir[9 * s + c] = VIR ? vals[1 + c] : 0.0;
   }
 }
 
@@ -539,9 +544,14 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st) {
                      e.d_mesh, e.d_cmesh, 0);
   hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
                      e.d_mesh, e.d_cmesh, 1);
-  hipLaunchKernelGGL(k_fft_xgreen, dim3(maxx, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
-                     e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
-                     e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
+  if (want_virial)
+    hipLaunchKernelGGL(k_fft_xgreen<true>, dim3(maxx, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+                       e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
+                       e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
+  else
+    hipLaunchKernelGGL(k_fft_xgreen<false>, dim3(maxx, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+                       e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
+                       e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
   hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
                      e.d_mesh, e.d_cmesh, 3);
   hipLaunchKernelGGL(k_fft_pass, dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,

@@ -164,35 +164,39 @@ static void to_soa(const double *aos, int n, std::vector<double> &soa) {
 }
 
 // ------------------------------------------------------------ RSpaceEwald
-extern "C" void RSpaceEwald(int nsite, int ngroup, const int (*group)[2], const cart_t *r,
-                            enum bcondtype_t bctype, const tensor_t *latv,
-                            int *const *neighbors, const int *number_of_neighbors,
-                            const struct PairBinary *group_excluded,
-                            const struct PairBinary *excluded, double rlo2, double rhi2,
-                            const int *site_mask, const double *q, double *phi, cart_t *evec,
-                            double eta, const struct Chebyshev *rspace_chebyshev,
-                            const double *rscreen, const double *sig, const double *eps,
-                            int geometric_combining_for_sigma, double *lj_ener, cart_t *lj_force,
-                            const struct PairBinary *one_four, double elec_1_4_scale_factor,
-                            double lj_1_4_scale_factor) {
-  (void)rspace_chebyshev;
-  (void)elec_1_4_scale_factor;
-  (void)lj_1_4_scale_factor;
-  if (bctype == non_periodic) die_msg("RSpaceEwald", "bctype != non_periodic (rspace.c:409)");
-  if (!(rhi2 >= rlo2) || !(rlo2 > 0)) die_msg("RSpaceEwald", "need rhi2 >= rlo2 > 0 (rspace.c:410-411)");
+// the checks RSpaceEwald and RSpaceEwaldPhi share (rspace.c:409-411 and
+// the inputs the device path does not cover)
+static void rspace_checks(const char *who, int nsite, int ngroup, enum bcondtype_t bctype,
+                          const tensor_t *latv, const struct PairBinary *group_excluded,
+                          double rlo2, double rhi2, const double *rscreen,
+                          const struct PairBinary *one_four) {
+  if (bctype == non_periodic) die_msg(who, "bctype != non_periodic (rspace.c:409)");
+  if (!(rhi2 >= rlo2) || !(rlo2 > 0)) die_msg(who, "need rhi2 >= rlo2 > 0 (rspace.c:410-411)");
   if (bctype != cubic || latv->yy != latv->xx || latv->zz != latv->xx || latv->xy != 0 ||
       latv->xz != 0 || latv->yx != 0 || latv->yz != 0 || latv->zx != 0 || latv->zy != 0)
-    die_msg("RSpaceEwald", "the device path covers cubic boxes only");
+    die_msg(who, "the device path covers cubic boxes only");
   for (int i = 0; rscreen && i < nsite; i++)
-    if (rscreen[i] != 0) die_msg("RSpaceEwald", "screening radii are not supported on the device");
+    if (rscreen[i] != 0) die_msg(who, "screening radii are not supported on the device");
   if (one_four)
     for (int i = 0; i < nsite; i++)
       if (PairBinary_number_of_elements(one_four)[i])
-        die_msg("RSpaceEwald", "1-4 scaled pairs are not supported on the device");
+        die_msg(who, "1-4 scaled pairs are not supported on the device");
   if (group_excluded)
     for (int g = 0; g < ngroup; g++)
       if (PairBinary_number_of_elements(group_excluded)[g])
-        die_msg("RSpaceEwald", "exclusions between groups are not supported on the device");
+        die_msg(who, "exclusions between groups are not supported on the device");
+}
+
+// Real-space sums over the caller's neighbour list on the device: a
+// transient single-system engine fed the caller's rows (as full rows),
+// the pair kernel in field mode and the excluded-pair loop.  out: 7 per
+// site (phi, evec, lj_force) and the LJ energy at out[7 nsite].
+static void rspace_on_device(const char *who, int nsite, int ngroup, const int (*group)[2],
+                             const cart_t *r, const tensor_t *latv, int *const *neighbors,
+                             const int *number_of_neighbors, const struct PairBinary *excluded,
+                             double rlo2, double rhi2, const int *site_mask, const double *q,
+                             const double *sig, const double *eps, int geometric, double eta,
+                             int min_image, std::vector<double> &out) {
   SystemInput S;
   S.nsite = nsite;
   S.ngroup = ngroup;
@@ -222,9 +226,9 @@ extern "C" void RSpaceEwald(int nsite, int ngroup, const int (*group)[2], const 
   S.nalias = 0;
   S.ik = 0;
   S.include_lj_correction = 0;
-  S.geometric_sigma = geometric_combining_for_sigma;
+  S.geometric_sigma = geometric;
   Engine e;
-  if (engine_create(e, {S})) die_msg("RSpaceEwald", last_error());
+  if (engine_create(e, {S})) die_msg(who, last_error());
   e.external_nl = 1;
   // caller's neighbour rows as full rows in the caller's group order
   std::vector<std::vector<int>> rows(ngroup);
@@ -252,16 +256,39 @@ extern "C" void RSpaceEwald(int nsite, int ngroup, const int (*group)[2], const 
   std::vector<double> soa;
   to_soa((const double *)r, nsite, soa);
   HIPCHK(hipMemcpy(e.d_r, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
-  std::vector<double> out(7 * (size_t)nsite + 1);
+  out.assign(7 * (size_t)nsite + 1, 0.0);
   double *d_out = nullptr;
   HIPCHK(hipMalloc((void **)&d_out, sizeof(double) * out.size()));
   HIPCHK(hipMemset(d_out, 0, sizeof(double) * out.size()));
-  if (launch_pack(e) || launch_pair(e, 1, 0) || launch_field_out(e, d_out, d_out + 7 * (size_t)nsite))
-    die_msg("RSpaceEwald", last_error());
+  if (launch_pack(e) || launch_pair(e, 1, 0) ||
+      launch_field_out(e, d_out, d_out + 7 * (size_t)nsite, min_image))
+    die_msg(who, last_error());
   HIPCHK(hipStreamSynchronize(e.stream));
   HIPCHK(hipMemcpy(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost));
   HIPCHK(hipFree(d_out));
   engine_destroy(e);
+}
+
+extern "C" void RSpaceEwald(int nsite, int ngroup, const int (*group)[2], const cart_t *r,
+                            enum bcondtype_t bctype, const tensor_t *latv,
+                            int *const *neighbors, const int *number_of_neighbors,
+                            const struct PairBinary *group_excluded,
+                            const struct PairBinary *excluded, double rlo2, double rhi2,
+                            const int *site_mask, const double *q, double *phi, cart_t *evec,
+                            double eta, const struct Chebyshev *rspace_chebyshev,
+                            const double *rscreen, const double *sig, const double *eps,
+                            int geometric_combining_for_sigma, double *lj_ener, cart_t *lj_force,
+                            const struct PairBinary *one_four, double elec_1_4_scale_factor,
+                            double lj_1_4_scale_factor) {
+  (void)rspace_chebyshev;
+  (void)elec_1_4_scale_factor;
+  (void)lj_1_4_scale_factor;
+  rspace_checks("RSpaceEwald", nsite, ngroup, bctype, latv, group_excluded, rlo2, rhi2, rscreen,
+                one_four);
+  std::vector<double> out;
+  rspace_on_device("RSpaceEwald", nsite, ngroup, group, r, latv, neighbors, number_of_neighbors,
+                   excluded, rlo2, rhi2, site_mask, q, sig, eps, geometric_combining_for_sigma,
+                   eta, 1, out);
   for (int i = 0; i < nsite; i++) {
     phi[i] += out[7 * (size_t)i];
     evec[i].x += out[7 * (size_t)i + 1];
@@ -272,6 +299,90 @@ extern "C" void RSpaceEwald(int nsite, int ngroup, const int (*group)[2], const 
     lj_force[i].z += out[7 * (size_t)i + 6];
   }
   *lj_ener += out[7 * (size_t)nsite];
+}
+
+// phi only (rspace.c:649-782): the smoothed pair potentials of RSpaceEwald
+// and the excluded-pair correction without the minimum image (:747-751)
+extern "C" void RSpaceEwaldPhi(int nsite, int ngroup, const int (*group)[2], const cart_t *r,
+                               enum bcondtype_t bctype, const tensor_t *latv,
+                               int *const *neighbors, const int *number_of_neighbors,
+                               const struct PairBinary *group_excluded,
+                               const struct PairBinary *excluded, double rlo2, double rhi2,
+                               const int *site_mask, const double *q, double *phi, double eta,
+                               const struct Chebyshev *rspace_chebyshev, const double *rscreen,
+                               const struct PairBinary *one_four, double elec_1_4_scale_factor) {
+  (void)rspace_chebyshev;
+  (void)elec_1_4_scale_factor;
+  rspace_checks("RSpaceEwaldPhi", nsite, ngroup, bctype, latv, group_excluded, rlo2, rhi2,
+                rscreen, one_four);
+  std::vector<int> qmask(nsite);
+  for (int i = 0; i < nsite; i++) qmask[i] = site_mask[i] & SITE_HAS_Q;   // no LJ
+  const std::vector<double> zero(nsite, 0.0);
+  std::vector<double> out;
+  rspace_on_device("RSpaceEwaldPhi", nsite, ngroup, group, r, latv, neighbors,
+                   number_of_neighbors, excluded, rlo2, rhi2, qmask.data(), q, zero.data(),
+                   zero.data(), 0, eta, 0, out);
+  for (int i = 0; i < nsite; i++) phi[i] += out[7 * (size_t)i];
+}
+
+// potential at grid points from all sites (rspace.c:892-967)
+extern "C" void RSpaceEwaldPhiAtGridpoints(int nsite, int ngrid, const cart_t *rsite,
+                                           const cart_t *rgrid, enum bcondtype_t bctype,
+                                           const tensor_t *latv, double rlo2, double rhi2,
+                                           const int *site_mask, const double *q, double *phi,
+                                           double eta, const struct Chebyshev *rspace_chebyshev,
+                                           const double *rscreen) {
+  (void)rspace_chebyshev;
+  const char *who = "RSpaceEwaldPhiAtGridpoints";
+  rspace_checks(who, nsite, 0, bctype, latv, nullptr, rlo2, rhi2, rscreen, nullptr);
+  if (ngrid <= 0 || nsite <= 0) return;
+  // a one-site-per-group system supplies the table and box parameters
+  SystemInput S;
+  S.nsite = nsite;
+  S.ngroup = nsite;
+  for (int i = 0; i < nsite; i++) {
+    S.gfirst.push_back(i);
+    S.gns.push_back(1);
+  }
+  S.q.assign(nsite, 0.0);
+  S.sig.assign(nsite, 0.0);
+  S.eps.assign(nsite, 0.0);
+  S.mask.assign(nsite, 0);
+  S.boxl = latv->xx;
+  S.rc = sqrt(rhi2);
+  S.width = S.rc - sqrt(rlo2);
+  S.skin = 1e-3;
+  S.eta = eta;
+  S.kmode = 2;
+  S.kc = 0;
+  S.nx = S.ny = S.nz = 0;
+  S.order = 1;
+  S.nalias = 0;
+  S.ik = 0;
+  S.include_lj_correction = 0;
+  S.geometric_sigma = 0;
+  Engine e;
+  if (engine_create(e, {S})) die_msg(who, last_error());
+  std::vector<double4> sites(nsite);
+  for (int i = 0; i < nsite; i++)
+    sites[i] = make_double4(rsite[i].x, rsite[i].y, rsite[i].z, site_mask[i] ? q[i] : 0.0);
+  double4 *d_sites = nullptr;
+  double *d_pts = nullptr, *d_phi = nullptr;
+  HIPCHK(hipMalloc((void **)&d_sites, sizeof(double4) * nsite));
+  HIPCHK(hipMalloc((void **)&d_pts, sizeof(double) * 3 * (size_t)ngrid));
+  HIPCHK(hipMalloc((void **)&d_phi, sizeof(double) * ngrid));
+  HIPCHK(hipMemcpy(d_sites, sites.data(), sizeof(double4) * nsite, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_pts, rgrid, sizeof(double) * 3 * (size_t)ngrid, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(d_phi, 0, sizeof(double) * ngrid));
+  if (launch_phi_points(e, d_sites, nsite, d_pts, ngrid, d_phi)) die_msg(who, last_error());
+  HIPCHK(hipStreamSynchronize(e.stream));
+  std::vector<double> out(ngrid);
+  HIPCHK(hipMemcpy(out.data(), d_phi, sizeof(double) * ngrid, hipMemcpyDeviceToHost));
+  hipFree(d_sites);
+  hipFree(d_pts);
+  hipFree(d_phi);
+  engine_destroy(e);
+  for (int j = 0; j < ngrid; j++) phi[j] += out[j];
 }
 
 // ------------------------------------------------------------ k space

@@ -16,7 +16,9 @@
 //
 // Differences from the reference, by design: attempts and velocity redraws
 // happen at fixed intervals instead of with probability 1/interval per step
-// (do_every, mdyn.cpp:49-52); forces are recomputed after a swap (the
+// (do_every, mdyn.cpp:49-52); the redraw follows the exchange, so swapped
+// rigid molecules get velocities constrained for the configuration they
+// now hold; forces are recomputed after a swap (the
 // reference half-kicks once with the forces of the configuration it held
 // before the swap); random numbers come from a counter-based generator
 // (splitmix64 of seed, attempt, index).

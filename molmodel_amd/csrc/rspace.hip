@@ -377,6 +377,79 @@ int launch_pack(Engine &e) {
   return 0;
 }
 
+// Real-space potential at arbitrary points (RSpaceEwaldPhiAtGridpoints,
+// rspace.c:892-967): one thread per point, the sites streamed through LDS
+// in tiles; minimum image, cutoff and switching on the site-point distance
+// in FP64 (the reference's decision), erfc(eta r)/r from the LDS table in
+// FP32, accumulation in FP64.  sites: (x, y, z, q) with q = 0 for sites
+// without any mask bit.
+constexpr int PHI_TB = 256;
+
+__global__ void __launch_bounds__(PHI_TB) k_phi_points(const SysParams *__restrict__ sp,
+                                                       const double4 *__restrict__ sites, int nsite,
+                                                       const double *__restrict__ pts, int npts,
+                                                       const float4 *__restrict__ tab,
+                                                       double *__restrict__ phi) {
+  __shared__ float4 stab[MAXTAB];
+  __shared__ double4 ssite[PHI_TB];
+  const SysParams &P = sp[0];
+  const int ntab = P.ntabA + P.ntabB;
+  for (int t = threadIdx.x; t < ntab; t += PHI_TB) stab[t] = tab[P.tab0 + t];
+  const int j = blockIdx.x * PHI_TB + threadIdx.x;
+  const bool on = j < npts;
+  const double gx = on ? pts[3 * j] : 0, gy = on ? pts[3 * j + 1] : 0, gz = on ? pts[3 * j + 2] : 0;
+  const double L = P.L, invL = P.invL, rhi2 = P.rhi2, rlo2 = P.rlo2;
+  const float tAmin = P.tabAmin, tAinv = P.tabAinv, tBinv = P.tabBinv, tBmax = P.tabBmax;
+  const float offA = -tAmin * tAinv, offB = (float)P.ntabA - P.tabMid * tBinv;
+  const float eta = (float)P.eta;
+  double acc = 0;
+  for (int t0 = 0; t0 < nsite; t0 += PHI_TB) {
+    __syncthreads();
+    if (t0 + (int)threadIdx.x < nsite) ssite[threadIdx.x] = sites[t0 + threadIdx.x];
+    __syncthreads();
+    const int m = min(PHI_TB, nsite - t0);
+    if (!on) continue;
+    for (int k = 0; k < m; k++) {
+      const double4 s = ssite[k];
+      double x = s.x - gx, y = s.y - gy, z = s.z - gz;
+      x -= L * floor(x * invL + 0.5);
+      y -= L * floor(y * invL + 0.5);
+      z -= L * floor(z * invL + 0.5);
+      const double r2 = x * x + y * y + z * z;
+      if (r2 > rhi2 || s.w == 0.0) continue;
+      const float r2f = (float)r2;
+      float b0;
+      if (r2f >= tAmin && r2f < tBmax) {
+        const float xa = fmaf(r2f, tAinv, offA), xb = fmaf(r2f, tBinv, offB);
+        const float xx = xa < xb ? xa : xb;
+        const int ii = (int)xx;
+        const float fr = xx - (float)ii;
+        const float4 c = stab[ii];
+        b0 = fmaf(fmaf(fmaf(c.w, fr, c.z), fr, c.y), fr, c.x);
+      } else {
+        const float rm = sqrtf(r2f);
+        b0 = erfcf(eta * rm) / rm;
+      }
+      double smooth = 1.0;
+      if (r2 > rlo2) {   // rspace.c:52-58
+        const double zz = (r2 - rlo2) * P.inv_w2;
+        smooth = ((-6 * zz + 15) * zz - 10) * zz * zz * zz + 1;
+      }
+      acc += s.w * (double)b0 * smooth;
+    }
+  }
+  if (on) phi[j] += acc;
+}
+
+int launch_phi_points(Engine &e, const double4 *sites, int nsite, const double *pts, int npts,
+                      double *phi) {
+  if (npts <= 0) return 0;
+  hipLaunchKernelGGL(k_phi_points, dim3((npts + PHI_TB - 1) / PHI_TB), dim3(PHI_TB), 0, e.stream,
+                     e.d_sp, sites, nsite, pts, npts, e.d_tab, phi);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_pair(Engine &e, int field_mode, int want_virial) {
   dim3 grid((e.maxg + GPB - 1) / GPB, e.nsys), tb(PAIR_TB);
   if (field_mode)

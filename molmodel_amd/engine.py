@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmolmodel_amd.so")
+# MOLMODEL_AMD_LIB selects another build of the same library (e.g. one
+# compiled with different kernel parameters); there is no other fallback
+LIB_PATH = os.environ.get("MOLMODEL_AMD_LIB", os.path.join(HERE, "libmolmodel_amd.so"))
 
 _P = C.c_void_p
 _D = C.c_double

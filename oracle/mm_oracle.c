@@ -435,15 +435,15 @@ static void interact_ewald(pairctx *c, int i, int j, double x, double y, double 
 }
 
 /* rspace.c:382-558 for cubic boundary conditions */
-void orc_rspace_ewald(int nsite, int ngroup, const int *group, const double *r,
-                      double boxl, long nnl, const int *nl,
-                      int nexcl, const int *excl, double rlo2, double rhi2,
-                      const int *site_mask, const double *q, double eta,
-                      const orc_cheby_t *cheb, const double *rscreen,
-                      const double *sig, const double *eps, int geometric_sigma,
-                      int n14, const int *one_four, double s14e, double s14lj,
-                      double *phi, double *evec, double *lj_ener, double *lj_force,
-                      double *virial)
+static void rspace_impl(int nsite, int ngroup, const int *group, const double *r,
+                        double boxl, long nnl, const int *nl,
+                        int nexcl, const int *excl, double rlo2, double rhi2,
+                        const int *site_mask, const double *q, double eta,
+                        const orc_cheby_t *cheb, const double *rscreen,
+                        const double *sig, const double *eps, int geometric_sigma,
+                        int n14, const int *one_four, double s14e, double s14lj,
+                        double *phi, double *evec, double *lj_ener, double *lj_force,
+                        double *virial, int excl_min_image)
 {
   const double negb = -boxl, invb = 1 / boxl;
   pairctx c = { phi, evec, lj_ener, lj_force, virial, q, sig, eps, rscreen, cheb, eta,
@@ -533,9 +533,11 @@ void orc_rspace_ewald(int nsite, int ngroup, const int *group, const double *r,
       const int j = ecol[t];
       if (!(site_mask[j] & ORC_SITE_HAS_Q)) continue;
       double x = r[3 * i] - r[3 * j], y = r[3 * i + 1] - r[3 * j + 1], z = r[3 * i + 2] - r[3 * j + 2];
-      x += negb * floor(x * invb + 0.5);
-      y += negb * floor(y * invb + 0.5);
-      z += negb * floor(z * invb + 0.5);
+      if (excl_min_image) {   /* RSpaceEwaldPhi omits it (rspace.c:747-751) */
+        x += negb * floor(x * invb + 0.5);
+        y += negb * floor(y * invb + 0.5);
+        z += negb * floor(z * invb + 0.5);
+      }
       const double r2 = x * x + y * y + z * z;
       const double rm = sqrt(r2), etar = eta * rm;
       const double ex = exp(-sq(etar));
@@ -574,6 +576,85 @@ void orc_rspace_ewald(int nsite, int ngroup, const int *group, const double *r,
     free(orow); free(ocol);
   }
   free(erow); free(ecol); free(gfor); free(grow); free(gcol);
+}
+
+void orc_rspace_ewald(int nsite, int ngroup, const int *group, const double *r,
+                      double boxl, long nnl, const int *nl,
+                      int nexcl, const int *excl, double rlo2, double rhi2,
+                      const int *site_mask, const double *q, double eta,
+                      const orc_cheby_t *cheb, const double *rscreen,
+                      const double *sig, const double *eps, int geometric_sigma,
+                      int n14, const int *one_four, double s14e, double s14lj,
+                      double *phi, double *evec, double *lj_ener, double *lj_force,
+                      double *virial)
+{
+  rspace_impl(nsite, ngroup, group, r, boxl, nnl, nl, nexcl, excl, rlo2, rhi2, site_mask, q,
+              eta, cheb, rscreen, sig, eps, geometric_sigma, n14, one_four, s14e, s14lj, phi,
+              evec, lj_ener, lj_force, virial, 1);
+}
+
+/* RSpaceEwaldPhi (rspace.c:649-782): the same smoothed pair potentials with
+ * charges only, and the excluded-pair correction without the minimum image.
+ * One-four pairs are not restated (the device path rejects them). */
+void orc_rspace_ewald_phi(int nsite, int ngroup, const int *group, const double *r,
+                          double boxl, long nnl, const int *nl, int nexcl, const int *excl,
+                          double rlo2, double rhi2, const int *site_mask, const double *q,
+                          double eta, const orc_cheby_t *cheb, const double *rscreen,
+                          double *phi)
+{
+  int *qmask = malloc(sizeof(int) * (size_t)(nsite > 0 ? nsite : 1));
+  double *ev = calloc(3 * (size_t)(nsite > 0 ? nsite : 1), sizeof(double));
+  double *ljf = calloc(3 * (size_t)(nsite > 0 ? nsite : 1), sizeof(double));
+  double *zero = calloc((size_t)(nsite > 0 ? nsite : 1), sizeof(double));
+  double lje = 0;
+  int i;
+  for (i = 0; i < nsite; i++) qmask[i] = site_mask[i] & ORC_SITE_HAS_Q;
+  rspace_impl(nsite, ngroup, group, r, boxl, nnl, nl, nexcl, excl, rlo2, rhi2, qmask, q, eta,
+              cheb, rscreen, zero, zero, 0, 0, NULL, 1, 1, phi, ev, &lje, ljf, NULL, 0);
+  free(qmask); free(ev); free(ljf); free(zero);
+}
+
+/* RSpaceEwaldPhiAtGridpoints (rspace.c:892-967): every site with any mask
+ * bit against every grid point, minimum image, cutoff and switching on the
+ * site-grid distance, the table branch inside (min, max), screening. */
+void orc_rspace_phi_at_gridpoints(int nsite, int ngrid, const double *rsite,
+                                  const double *rgrid, double boxl, double rlo2, double rhi2,
+                                  const int *site_mask, const double *q, double eta,
+                                  const orc_cheby_t *cheb, const double *rscreen, double *phi)
+{
+  const double negb = -boxl, invb = 1 / boxl;
+  int i, j;
+  for (i = 0; i < nsite; i++) {
+    if (!site_mask[i]) continue;
+    const double xi = rsite[3 * i], yi = rsite[3 * i + 1], zi = rsite[3 * i + 2];
+    const double qi = q[i];
+    const double rscr = rscreen ? rscreen[i] : 0, rscr2 = rscr * rscr;
+    for (j = 0; j < ngrid; j++) {
+      double x = xi - rgrid[3 * j], y = yi - rgrid[3 * j + 1], z = zi - rgrid[3 * j + 2];
+      x += negb * floor(x * invb + 0.5);
+      y += negb * floor(y * invb + 0.5);
+      z += negb * floor(z * invb + 0.5);
+      const double r2 = x * x + y * y + z * z;
+      if (r2 > rhi2) continue;
+      double smooth = 1, dsmooth = 0;
+      const int smoothing = r2 > rlo2;
+      if (smoothing) switching(r2, rlo2, rhi2, &smooth, &dsmooth);
+      double b0, b1;
+      if (cheb && cheb->min < r2 && r2 < cheb->max) {
+        orc_cheby_eval(cheb, r2, &b0, &b1);
+      } else {
+        const double rm = sqrt(r2);
+        b0 = erfc(eta * rm) / rm;
+      }
+      if (r2 < rscr2) {   /* screening_function (rspace.c:61-64) */
+        const double invr = 1.0 / sqrt(r2);
+        const double b1s = 1.0 / (rscr * rscr * rscr);
+        b0 += (-0.5 * r2 * b1s + 1.5 / rscr) - invr;
+      }
+      if (smoothing) b0 *= smooth;
+      phi[j] += qi * b0;
+    }
+  }
 }
 
 /* ------------------------------------------------------------------ */

@@ -81,6 +81,21 @@ void orc_rspace_ewald(int nsite, int ngroup, const int *group, const double *r,
                       double *phi, double *evec, double *lj_ener, double *lj_force,
                       double *virial);
 
+/* RSpaceEwaldPhi (rspace.c:649-782): phi only; excluded pairs without the
+ * minimum image (:747-751).  Accumulated (+=). */
+void orc_rspace_ewald_phi(int nsite, int ngroup, const int *group, const double *r,
+                          double boxl, long nnl, const int *nl, int nexcl, const int *excl,
+                          double rlo2, double rhi2, const int *site_mask, const double *q,
+                          double eta, const orc_cheby_t *cheb, const double *rscreen,
+                          double *phi);
+
+/* RSpaceEwaldPhiAtGridpoints (rspace.c:892-967): phi at ngrid points
+ * (AoS rgrid) from all sites with any mask bit.  Accumulated (+=). */
+void orc_rspace_phi_at_gridpoints(int nsite, int ngrid, const double *rsite,
+                                  const double *rgrid, double boxl, double rlo2, double rhi2,
+                                  const int *site_mask, const double *q, double eta,
+                                  const orc_cheby_t *cheb, const double *rscreen, double *phi);
+
 /* ---- Plain Ewald k-space (kspace.c:35-174) ---- */
 int orc_kspace_count(double kc);
 /* integer vectors in the reference order (sorted by |n|^2 descending) */

@@ -132,6 +132,52 @@ void ref_rspace(int nsite, int ngroup, const int *group, const double *r, double
   delete bc;
 }
 
+/* RSpaceEwaldPhiAtGridpoints as ElecLJ calls it (eleclj.cpp:1317-1321) */
+void ref_phi_at_gridpoints(int nsite, int ngrid, const double *rsite, const double *rgrid,
+                           double boxl, double rc, double smoothing_width, const int *site_mask,
+                           const double *q, double eta, double *phi)
+{
+  struct Chebyshev *cheb = RSpaceChebyshev_new(rc, eta, 0, 0);
+  double *rscreen = new double[nsite > 0 ? nsite : 1]();
+  tensor_t latv = { boxl, 0, 0, 0, boxl, 0, 0, 0, boxl };
+  RSpaceEwaldPhiAtGridpoints(nsite, ngrid, (const cart_t *)rsite, (const cart_t *)rgrid, cubic,
+                             &latv, (rc - smoothing_width) * (rc - smoothing_width), rc * rc,
+                             site_mask, q, phi, eta, cheb, rscreen);
+  free(cheb);
+  delete[] rscreen;
+}
+
+/* RSpaceEwaldPhi on the reference's own neighbour list (rspace.c:649-782) */
+void ref_rspace_phi(int nsite, int ngroup, const int *group, const double *r, double boxl,
+                    double rc, double smoothing_width, double skin, int nexcl, const int *excl,
+                    const int *site_mask, const double *q, double eta, double *phi)
+{
+  BoundaryConditions *bc = BoundaryConditions::new_cubic(boxl);
+  NeighborList nl(ngroup, rc, skin);
+  nl.set_boundary_conditions(*bc);
+  CVec gr(ngroup);
+  for (int g = 0; g < ngroup; g++) {
+    const int s = group[2 * g];
+    gr[g] = Cartesian(r[3 * s], r[3 * s + 1], r[3 * s + 2]);
+  }
+  nl.coordinates_changed(gr);
+  struct PairBinary *excluded = pb_from(nsite, nexcl, excl);
+  struct PairBinary *gexcl = PairBinary_new(ngroup);
+  PairBinary_init(gexcl);
+  struct Chebyshev *cheb = RSpaceChebyshev_new(rc, eta, 0, 0);
+  double *rscreen = new double[nsite > 0 ? nsite : 1]();
+  tensor_t latv = { boxl, 0, 0, 0, boxl, 0, 0, 0, boxl };
+  RSpaceEwaldPhi(nsite, ngroup, (const int(*)[2])group, (const cart_t *)r, cubic, &latv,
+                 nl.neighbors(), nl.number_of_neighbors(), gexcl, excluded,
+                 (rc - smoothing_width) * (rc - smoothing_width), rc * rc, site_mask, q, phi,
+                 eta, cheb, rscreen, 0, 1);
+  free(cheb);
+  delete[] rscreen;
+  PairBinary_delete(excluded);
+  PairBinary_delete(gexcl);
+  delete bc;
+}
+
 /* kspace_new + kspace_field for a cubic box (recip = 2 pi / L on the
  * diagonal, as BoundaryConditions::reciprocal_lattice_vectors gives). */
 void ref_kspace(double kc, int nsite, const double *r, const double *q, const double *recip,

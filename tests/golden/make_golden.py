@@ -17,6 +17,12 @@ the reference's outputs):
                      call that exercises the skin test
   pbin_kat.json      the PairBinary known-answer sequence of pbin.c:176-213
                      run through the reference, with the expected rows
+  ref_water648_phi.npz  the same box with the first hydrogen of ten waters
+                     moved by one box length (molecules straddling the edge,
+                     as unwrapped-but-folded coordinates leave them):
+                     RSpaceEwaldPhi phi and RSpaceEwald phi / evec there
+  ref_esp_grid.npz   RSpaceEwaldPhiAtGridpoints on the 648-site box at 400
+                     seeded points (inside and outside the central box)
 """
 import hashlib
 import json
@@ -37,10 +43,40 @@ def digest(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
+def wrapped_water648():
+    s, p = W.config("water648_ewald")
+    r = s.r.copy()
+    for w in range(10):
+        r[3 * w + 1, w % 3] += s.boxl if w % 2 == 0 else -s.boxl
+    return s, p, r
+
+
+def write_phi_fixture():
+    s, p, r = wrapped_water648()
+    phi_only = O.ref_rspace_phi(s, p, r=r)
+    phi, ev, _, _ = O.ref_rspace_parts(s, p, r=r)
+    np.savez_compressed(os.path.join(HERE, "ref_water648_phi.npz"), r=r, phi_only=phi_only,
+                        phi=phi, evec=ev)
+
+
+def esp_grid_points(boxl):
+    u = W.uniform(4242, 0, 3 * 400).reshape(400, 3)
+    return (u * 1.2 - 0.1) * boxl
+
+
+def write_esp_fixture():
+    s, p = W.config("water648_ewald")
+    grid = esp_grid_points(s.boxl)
+    phi = O.ref_phi_at_gridpoints(s, p, grid)
+    np.savez_compressed(os.path.join(HERE, "ref_esp_grid.npz"), grid=grid, phi=phi)
+
+
 def main():
     lib = O.ref()
     if lib is None:
         raise SystemExit("the reference build (oracle/_ref/libref.so) is not available here")
+    write_phi_fixture()
+    write_esp_fixture()
     s, p = W.config("water648_ewald")
     gr = O.first_sites(s)
     nl = O.ref_nl_pairs(gr, s.boxl, p["rc"], p["skin"])

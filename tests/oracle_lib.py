@@ -51,6 +51,10 @@ def oracle():
         lib.orc_rspace_ewald.argtypes = [_I, _I, _P, _P, _D, _L, _P, _I, _P, _D, _D, _P, _P,
                                          _D, _P, _P, _P, _P, _I, _I, _P, _D, _D,
                                          _P, _P, _P, _P, _P]
+        lib.orc_rspace_ewald_phi.argtypes = [_I, _I, _P, _P, _D, _L, _P, _I, _P, _D, _D, _P, _P,
+                                             _D, _P, _P, _P]
+        lib.orc_rspace_phi_at_gridpoints.argtypes = [_I, _I, _P, _P, _D, _D, _D, _P, _P, _D,
+                                                     _P, _P, _P]
         lib.orc_kspace_count.argtypes = [_D]
         lib.orc_kspace_vectors.argtypes = [_D, _P]
         lib.orc_kspace_field.argtypes = [_D, _I, _P, _P, _P, _D, _D, _P, _P, _P, _P]
@@ -94,6 +98,8 @@ def ref():
         lib.ref_rspace.argtypes = [_I, _I, _P, _P, _D, _D, _D, _D, _I, _P, _P, _P, _D, _P, _P,
                                    _P, _P, _P, _P]
         lib.ref_kspace.argtypes = [_D, _I, _P, _P, _P, _D, _D, _P, _P]
+        lib.ref_rspace_phi.argtypes = [_I, _I, _P, _P, _D, _D, _D, _D, _I, _P, _P, _P, _D, _P]
+        lib.ref_phi_at_gridpoints.argtypes = [_I, _I, _P, _P, _D, _D, _D, _P, _P, _D, _P]
         lib.ref_kspace_count.argtypes = [_D]
         lib.ref_pbin_kat.argtypes = [_P, _I, _I, _P, _P, _P]
         _ref = lib
@@ -183,6 +189,64 @@ def rspace_parts(system, p, r=None):
                          _ptr(system.sig), _ptr(system.eps), 0, 0, None, 1.0, 1.0,
                          _ptr(phi), _ptr(ev), _ptr(lje), _ptr(ljf), None)
     return phi, ev, ljf, lje[0]
+
+
+def rspace_phi(system, p, r=None):
+    """phi of RSpaceEwaldPhi alone (oracle, rspace.c:649-782)."""
+    lib = oracle()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    gr = first_sites(system, r)
+    pairs = nl_pairs(gr, system.boxl, p["rc"], p["skin"])
+    order = np.argsort(pairs[:, 0], kind="stable")
+    nl = np.ascontiguousarray(pairs[order])
+    cheb = (C.c_double * (3 + 512 * 7))()
+    lib.orc_cheby_rspace(cheb, p["rc"], p["eta"])
+    phi = np.zeros(system.nsite)
+    lib.orc_rspace_ewald_phi(system.nsite, system.ngroup,
+                             _ptr(np.ascontiguousarray(system.group)), _ptr(r), system.boxl,
+                             len(nl), _ptr(nl), len(system.excl),
+                             _ptr(np.ascontiguousarray(system.excl)),
+                             (p["rc"] - p["smoothing_width"]) ** 2, p["rc"] ** 2,
+                             _ptr(system.site_mask), _ptr(system.q), p["eta"], cheb, None,
+                             _ptr(phi))
+    return phi
+
+
+def phi_at_gridpoints(system, p, grid, r=None):
+    """RSpaceEwaldPhiAtGridpoints (oracle)."""
+    lib = oracle()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    grid = np.ascontiguousarray(grid, dtype=np.float64)
+    cheb = (C.c_double * (3 + 512 * 7))()
+    lib.orc_cheby_rspace(cheb, p["rc"], p["eta"])
+    phi = np.zeros(len(grid))
+    lib.orc_rspace_phi_at_gridpoints(system.nsite, len(grid), _ptr(r), _ptr(grid), system.boxl,
+                                     (p["rc"] - p["smoothing_width"]) ** 2, p["rc"] ** 2,
+                                     _ptr(system.site_mask), _ptr(system.q), p["eta"], cheb,
+                                     None, _ptr(phi))
+    return phi
+
+
+def ref_phi_at_gridpoints(system, p, grid, r=None):
+    lib = ref()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    grid = np.ascontiguousarray(grid, dtype=np.float64)
+    phi = np.zeros(len(grid))
+    lib.ref_phi_at_gridpoints(system.nsite, len(grid), _ptr(r), _ptr(grid), system.boxl, p["rc"],
+                              p["smoothing_width"], _ptr(system.site_mask), _ptr(system.q),
+                              p["eta"], _ptr(phi))
+    return phi
+
+
+def ref_rspace_phi(system, p, r=None):
+    lib = ref()
+    r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
+    phi = np.zeros(system.nsite)
+    lib.ref_rspace_phi(system.nsite, system.ngroup, _ptr(np.ascontiguousarray(system.group)),
+                       _ptr(r), system.boxl, p["rc"], p["smoothing_width"], p["skin"],
+                       len(system.excl), _ptr(np.ascontiguousarray(system.excl)),
+                       _ptr(system.site_mask), _ptr(system.q), p["eta"], _ptr(phi))
+    return phi
 
 
 def ref_rspace_parts(system, p, r=None):

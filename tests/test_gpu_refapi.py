@@ -2,6 +2,7 @@
 RSpaceEwald, kspace_new/kspace_field, p3m_new/p3m_init/p3m_field, called
 exactly as ElecLJ calls them (eleclj.cpp:932-950), against the FP64 oracle."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -78,6 +79,61 @@ def test_rspace_ewald_dropin(gpu_lib):
     print("dropin rspace", rel(phi, ophi), rel(ev, oev), rel(ljf, oljf), abs(lje[0] - olje) / abs(olje))
     assert rel(phi, ophi) < 2e-6 and rel(ev, oev) < 5e-6 and rel(ljf, oljf) < 5e-6
     assert abs(lje[0] - olje) < 1e-5 * abs(olje)
+
+
+def test_rspace_ewald_phi_dropin(gpu_lib):
+    """RSpaceEwaldPhi with molecules straddling the box edge (the excluded
+    correction without the minimum image) against the reference's output."""
+    L = lib()
+    L.RSpaceEwaldPhi.argtypes = [I, I, P, P, I, P, P, P, P, P, D, D, P, P, P, D, P, P, P, D]
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_water648_phi.npz"))
+    s, p = W.config("water648_ewald")
+    r = np.ascontiguousarray(g["r"])
+    gr = O.first_sites(s, r)
+    pairs = O.nl_pairs(gr, s.boxl, p["rc"], p["skin"])
+    rows = [[] for _ in range(s.ngroup)]
+    for a, b in pairs:
+        rows[a].append(b)
+    arrs = [np.array(x if x else [0], dtype=np.int32) for x in rows]
+    ptrs = (C.POINTER(C.c_int) * s.ngroup)(*[a.ctypes.data_as(C.POINTER(C.c_int)) for a in arrs])
+    nn = np.array([len(x) for x in rows], dtype=np.int32)
+    excl = L.PairBinary_new(s.nsite)
+    for a, b in s.excl:
+        L.PairBinary_add(excl, int(a), int(b))
+    L.PairBinary_init(excl)
+    gexcl = L.PairBinary_new(s.ngroup)
+    L.PairBinary_init(gexcl)
+    latv = np.array([s.boxl, 0, 0, 0, s.boxl, 0, 0, 0, s.boxl])
+    phi = np.zeros(s.nsite)
+    grp = np.ascontiguousarray(s.group)
+    L.RSpaceEwaldPhi(s.nsite, s.ngroup, grp.ctypes.data_as(P), r.ctypes.data_as(P), 1,
+                     latv.ctypes.data_as(P), C.cast(ptrs, P), nn.ctypes.data_as(P), gexcl, excl,
+                     (p["rc"] - p["smoothing_width"]) ** 2, p["rc"] ** 2,
+                     s.site_mask.ctypes.data_as(P), s.q.ctypes.data_as(P), phi.ctypes.data_as(P),
+                     p["eta"], None, None, None, 1.0)
+    L.PairBinary_delete(excl)
+    L.PairBinary_delete(gexcl)
+    print("RSpaceEwaldPhi rel", rel(phi, g["phi_only"]), "straddling sites",
+          rel(phi[:30], g["phi_only"][:30]))
+    assert rel(phi, g["phi_only"]) < 2e-6
+    assert rel(phi[:30], g["phi_only"][:30]) < 2e-6
+
+
+def test_phi_at_gridpoints_dropin(gpu_lib):
+    L = lib()
+    L.RSpaceEwaldPhiAtGridpoints.argtypes = [I, I, P, P, I, P, D, D, P, P, P, D, P, P]
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_esp_grid.npz"))
+    s, p = W.config("water648_ewald")
+    grid = np.ascontiguousarray(g["grid"])
+    latv = np.array([s.boxl, 0, 0, 0, s.boxl, 0, 0, 0, s.boxl])
+    phi = np.zeros(len(grid))
+    L.RSpaceEwaldPhiAtGridpoints(s.nsite, len(grid), s.r.ctypes.data_as(P), grid.ctypes.data_as(P),
+                                 1, latv.ctypes.data_as(P), (p["rc"] - p["smoothing_width"]) ** 2,
+                                 p["rc"] ** 2, s.site_mask.ctypes.data_as(P),
+                                 s.q.ctypes.data_as(P), phi.ctypes.data_as(P), p["eta"], None, None)
+    err = np.abs(phi - g["phi"]).max() / np.abs(g["phi"]).max()
+    print("phi at gridpoints max err / max |phi|", err, "rel", rel(phi, g["phi"]))
+    assert err < 2e-6
 
 
 def test_kspace_dropin(gpu_lib):

@@ -161,6 +161,41 @@ def test_rspace_matches_reference_fixture():
     assert lje == g["lj_energy"][0]
 
 
+def test_rspace_phi_matches_reference_fixture():
+    """RSpaceEwaldPhi (rspace.c:649-782) with ten molecules straddling the box
+    edge: the phi-only restatement equals the reference bit for bit, and it
+    differs from RSpaceEwald's phi exactly on those molecules' sites (the
+    excluded-pair correction skips the minimum image, :747-751)."""
+    g = gold("ref_water648_phi.npz")
+    s, p = W.config("water648_ewald")
+    r = g["r"]
+    phi_only = O.rspace_phi(s, p, r=r)
+    phi, ev, _, _ = O.rspace_parts(s, p, r=r)
+    assert np.array_equal(phi_only, g["phi_only"])
+    assert np.array_equal(phi, g["phi"]) and np.array_equal(ev, g["evec"])
+    differs = np.nonzero(np.abs(phi - phi_only) > 1e-12)[0]
+    assert differs.tolist() == list(range(30))
+
+
+def test_phi_at_gridpoints_matches_reference_fixture():
+    """RSpaceEwaldPhiAtGridpoints (rspace.c:892-967) at 400 points inside and
+    outside the central box: the restatement equals the reference bit for bit."""
+    g = gold("ref_esp_grid.npz")
+    s, p = W.config("water648_ewald")
+    assert np.array_equal(O.phi_at_gridpoints(s, p, g["grid"]), g["phi"])
+
+
+def test_phi_at_gridpoints_is_linear_in_charges():
+    """The potential at points scales linearly with the site charges."""
+    s, p = W.config("water216_small")
+    grid = W.uniform(5, 0, 3 * 50).reshape(50, 3) * s.boxl
+    a = O.phi_at_gridpoints(s, p, grid)
+    t = W.System(**{k: getattr(s, k) for k in s.__dataclass_fields__})
+    t.q = 2.5 * s.q
+    b = O.phi_at_gridpoints(t, p, grid)
+    assert np.allclose(b, 2.5 * a, rtol=1e-13, atol=1e-13)
+
+
 @pytest.mark.skipif(O.ref() is None, reason="reference sources not built in this environment")
 def test_rspace_matches_live_reference_other_seed():
     s, p = W.config("water648_ewald", seed=77)
