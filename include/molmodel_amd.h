@@ -132,6 +132,14 @@ long long mm_nl_builds(mm_engine *e);   /* neighbour-list builds so far, all sys
 int mm_md_setup(mm_engine *e, double dt, double bond, double angle_deg, double tol);
 int mm_md_run(mm_engine *e, int nsteps, int use_graph);
 
+/* Potential::scale(s) for one system as ElecLJ does it (eleclj.cpp:1268-1280,
+ * called by the volume move, mdyn.cpp:535, 558): box length times s, eta
+ * divided by s, P3M influence function divided by s (p3m_scale,
+ * pppm.c:643-664), wave vectors, erfc table, tail correction and
+ * neighbour-list cells for the new box.  Positions are not touched: the
+ * caller scales molecule centres and sets them (MSys::scale_centroids). */
+int mm_scale_system(mm_engine *e, int sys, double s);
+
 /* NeighborList (nbrlist.hpp:8-40) on the device for n points in a cubic box
  * of side boxl: NeighborList(n, cutoff, width) + set_boundary_conditions
  * (nbrlist.cpp:9-17, 84-123).  update = coordinates_changed(r)

@@ -473,6 +473,19 @@ long mm_nlist_rows(mm_nlist *nl, int *row_start, int *cols, long cap) {
   return np;
 }
 
+int mm_scale_system(mm_engine *m, int sys, double f) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  MM_HIP(hipStreamSynchronize(e.stream));
+  TRY(engine_scale_system(e, sys, f));
+  // the captured step holds the old stencil buffer
+  if (e.gexec) { hipGraphExecDestroy(e.gexec); e.gexec = nullptr; }
+  if (e.graph) { hipGraphDestroy(e.graph); e.graph = nullptr; }
+  m->pos_dirty = 1;
+  e.md_phase = 0;
+  return MM_OK;
+}
+
 int mm_md_setup(mm_engine *m, double dt, double bond, double angle_deg, double tol) {
   CHECK_ENGINE(m);
   Engine &e = m->e;

@@ -575,6 +575,88 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   return 0;
 }
 
+// ElecLJ::scale (eleclj.cpp:1268-1280) for one system: the box grows by f,
+// eta shrinks by f (so eta L is kept), the erfc table is rebuilt for the new
+// eta, the P3M influence function scales as G/f (p3m_scale, pppm.c:643-664;
+// exact, since G depends on k/eta and 1/V), the Ewald wave vectors and
+// coefficients and the neighbour-list cells (set_boundary_conditions,
+// nbrlist.cpp:84-123) follow the new box.  Positions are the caller's.
+int engine_scale_system(Engine &e, int s, double f) {
+  MM_REQ(s >= 0 && s < e.nsys, "bad system index");
+  MM_REQ(f > 0 && std::isfinite(f), "scale factor must be positive");
+  SystemInput &S = e.in[s];
+  SysParams &P = e.hsp[s];
+  MM_REQ(!S.has_latv, "box scaling covers cubic boxes");
+  const double L = S.boxl * f;
+  MM_REQ(S.rc <= 0.49 * L, "rspace cutoff must stay <= 0.49 * box length (eleclj.cpp:164-170)");
+  S.boxl = L;
+  S.eta /= f;
+  P.L = L;
+  P.invL = 1 / L;
+  P.volume = L * L * L;
+  P.eta = S.eta;
+  P.self_coef = S.eta * 1.12837916709551257390;
+  double qsum = 0;
+  for (double x : S.q) qsum += x;
+  P.bg_phi = fabs(qsum) < 64.0 * 2.220446049250313e-16 ? 0.0 : -M_PI / (sq(S.eta) * P.volume) * qsum;
+  P.lc = L * (1.0 / P.ncell);
+  // erfc table for the new eta: same intervals (they depend on rc only)
+  {
+    std::vector<float4> tab;
+    SysParams T = P;
+    build_table(T, S.rc, S.eta, tab);
+    MM_REQ(T.ntabA == P.ntabA && T.ntabB == P.ntabB, "erfc table size changed");
+    P.tabAmin = T.tabAmin; P.tabMid = T.tabMid; P.tabAinv = T.tabAinv; P.tabBinv = T.tabBinv;
+    P.tabBmax = T.tabBmax;
+    MM_HIP(hipMemcpy(e.d_tab + P.tab0, tab.data(), sizeof(float4) * tab.size(), hipMemcpyHostToDevice));
+  }
+  const double latv[9] = {L, 0, 0, 0, L, 0, 0, 0, L};
+  double inv[9];
+  tensor_inverse(inv, latv);
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) P.RL[3 * a + b] = 2 * M_PI * inv[3 * b + a];
+  if (P.kmode == 0) {   // kspace.c:96-103 with the new box and eta
+    std::vector<double> kv(4 * (size_t)P.nk);
+    MM_HIP(hipMemcpy(kv.data(), e.d_kvec + 4L * P.k0, sizeof(double) * kv.size(), hipMemcpyDeviceToHost));
+    const double c = 0.25 / (S.eta * S.eta), fourpi_V = 4 * M_PI / P.volume;
+    for (int k = 0; k < P.nk; k++) {
+      double *v = &kv[4 * (size_t)k];
+      v[0] /= f; v[1] /= f; v[2] /= f;
+      const double t = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+      v[3] = fourpi_V * exp(-t * c) / t;
+    }
+    MM_HIP(hipMemcpy(e.d_kvec + 4L * P.k0, kv.data(), sizeof(double) * kv.size(), hipMemcpyHostToDevice));
+  } else if (P.kmode == 1) {
+    for (int a = 0; a < 9; a++) P.T[a] = inv[a];
+    P.T[0] *= P.nx; P.T[3] *= P.nx; P.T[6] *= P.nx;
+    P.T[1] *= P.ny; P.T[4] *= P.ny; P.T[7] *= P.ny;
+    P.T[2] *= P.nz; P.T[5] *= P.nz; P.T[8] *= P.nz;
+    const size_t ng = (size_t)P.nx * P.ny * (P.nz / 2 + 1);
+    std::vector<float> g(ng);
+    MM_HIP(hipMemcpy(g.data(), e.d_green + P.green0, sizeof(float) * ng, hipMemcpyDeviceToHost));
+    for (auto &x : g) x = (float)((double)x / f);
+    MM_HIP(hipMemcpy(e.d_green + P.green0, g.data(), sizeof(float) * ng, hipMemcpyHostToDevice));
+  }
+  e.lj_tail[s] = S.include_lj_correction ? lj_tail(S) : 0.0;
+  // neighbour-list stencils of every system (this one may change length)
+  std::vector<int4> sten;
+  for (int t = 0; t < e.nsys; t++) {
+    SysParams &Q = e.hsp[t];
+    const SystemInput &U = e.in[t];
+    auto st = nl_stencil(Q.ncell, U.boxl, U.rc, U.skin);
+    Q.sten0 = (int)sten.size();
+    Q.nsten = (int)st.size();
+    sten.insert(sten.end(), st.begin(), st.end());
+  }
+  MM_HIP(hipFree(e.d_sten));
+  e.d_sten = nullptr;
+  MM_TRY(dalloc(&e.d_sten, sten.size()));
+  MM_TRY(dput(e.d_sten, sten));
+  MM_TRY(dput(e.d_sp, e.hsp));
+  e.first_eval = 1;   // cells changed: rebuild every list
+  return 0;
+}
+
 void engine_destroy(Engine &e) {
   if (e.gexec) hipGraphExecDestroy(e.gexec);
   if (e.graph) hipGraphDestroy(e.graph);

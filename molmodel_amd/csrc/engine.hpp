@@ -149,6 +149,7 @@ double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl
 void cheb_fit(double a, double b, double (*f)(double), double *c4);
 void engine_destroy(Engine &e);
 void lj_tail_factors(const SystemInput &s, double *ca, double *cb);
+int engine_scale_system(Engine &e, int s, double f);
 
 // launchers (each returns 0 or a negative status)
 int launch_nl_build(Engine &e, int force);
@@ -156,7 +157,7 @@ int launch_pack(Engine &e);
 int launch_pair(Engine &e, int field_mode, int want_virial);
 // mesh lines per FFT workgroup (16 threads per line); the x pass writes one
 // partial-sum slot per workgroup
-constexpr int P3M_FFT_LINES = 4;
+constexpr int P3M_FFT_LINES = 16;
 
 int launch_kspace(Engine &e, int want_virial, hipStream_t st);
 int launch_p3m(Engine &e, int want_virial, hipStream_t st);

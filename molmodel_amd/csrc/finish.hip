@@ -24,7 +24,7 @@
 
 namespace mmamd {
 
-constexpr int FIN_TB = 64;
+constexpr int FIN_TB = 256;
 // reduced per system: elec, lj, kin, [vir[9]], top1, top2
 template <bool VIR> struct FinSlots {
   static constexpr int N = VIR ? 14 : 5;

@@ -68,6 +68,7 @@ def lib():
         L.mm_remd_stats.argtypes = [_P, _P, _P]
         L.mm_tune_kspace.argtypes = [_D, _I, _D, _D, _I, _P]
         L.mm_param_gradients.argtypes = [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
+        L.mm_scale_system.argtypes = [_P, _I, _D]
         L.mm_nl_builds.restype = C.c_longlong
         L.mm_nl_builds.argtypes = [_P]
         _lib = L
