@@ -40,6 +40,7 @@ namespace {
 int put_q(Engine &e, int s, const std::vector<double> &q) {
   const SysParams &P = e.hsp[s];
   MM_HIP(hipMemcpy(e.d_q + P.site0, q.data(), sizeof(double) * P.nsite, hipMemcpyHostToDevice));
+  e.need_pack = 1;   // the packed site records carry the charges
   return 0;
 }
 

@@ -218,8 +218,9 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
     if (m->pos_dirty) TRY(launch_skin(e));
     TRY(launch_nl_build(e, 0));
   }
+  if (e.need_pack || m->pos_dirty) TRY(launch_pack(e));
+  e.need_pack = 0;
   m->pos_dirty = 0;
-  TRY(launch_pack(e));
   TRY(launch_pair(e, field_mode, want_virial));
   if (fork) MM_HIP(hipStreamWaitEvent(e.stream, e.ev_join, 0));
   return 0;
@@ -629,6 +630,7 @@ int mm_remd_run(mm_engine *m, int nintervals, int steps, int reinit_velocities, 
     // constrained for the configuration they left)
     TRY(launch_remd_exchange(e, a));
     if (reinit_velocities) TRY(launch_remd_velocities(e, a));
+    e.need_pack = 1;   // positions moved between systems
     // forces and energies of the configurations now held; swapped systems
     // rebuild their neighbour lists first
     TRY(enqueue_forces(m, 0, 0));

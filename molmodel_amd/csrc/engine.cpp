@@ -468,6 +468,7 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
       }
       green.insert(green.end(), gh.begin(), gh.end());
       const int nn[3] = {S.nx, S.ny, S.nz};
+      P.twid0 = (int)twid.size();
       for (int d = 0; d < 3; d++)
         for (int t = 0; t < 128; t++) {
           const double a = -2 * M_PI * (t % nn[d]) / nn[d];

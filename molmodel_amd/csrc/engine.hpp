@@ -103,7 +103,7 @@ struct Engine {
   float *d_mesh = nullptr;
   float2 *d_cmesh = nullptr;
   float *d_green = nullptr;
-  float2 *d_twid = nullptr;   // per system: twiddles for nx, ny, nz (3*128 entries)
+  float2 *d_twid = nullptr;   // per P3M system (at SysParams::twid0): twiddles for nx, ny, nz (3*128)
   int4 *d_sgrid = nullptr;    // per site: jx jy jz bin
   float4 *d_sfrac = nullptr;  // per site: sx sy sz q
   int nbin_total = 0;
@@ -126,6 +126,10 @@ struct Engine {
   double *d_phi = nullptr;   // per-site total potential of the last evaluation
   double *d_scratch = nullptr; size_t scratch_n = 0;
   unsigned long long *d_nlb = nullptr;   // neighbour-list builds per system
+  // the pair kernel's packed inputs are refreshed by k_finish's drift and by
+  // a neighbour-list rebuild; anything else that changes positions or
+  // charges sets this so the next evaluation packs explicitly
+  int need_pack = 1;
   // replica exchange: ladders of system indices by ascending temperature
   // (separated by -1), kT per system, per-attempt permutation, which
   // starting configuration each system holds, tries/accepts per system

@@ -56,6 +56,7 @@ struct SysParams {
   double T[9];                 // diag(n) L^-1, tensor_t layout (pppm.c:523-525)
   double RL[9];                // reciprocal lattice 2 pi (L^-1)^T, tensor_t layout
   int bx, by, nbx, nby, bin0;  // spread column blocks (bx x by mesh columns each)
+  int twid0;                   // first of this system's 3 x 128 FFT twiddles
   // dynamics
   double dt;
   double bg_phi;               // neutralising background potential (or 0)

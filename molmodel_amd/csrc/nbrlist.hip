@@ -106,7 +106,9 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
     const int *__restrict__ order, const int *__restrict__ gcell,
     const int *__restrict__ cell_start, const double *__restrict__ last,
     const int4 *__restrict__ sten, int nbcap, int *__restrict__ nbr, int *__restrict__ nnb,
-    int *__restrict__ err) {
+    int *__restrict__ err, const double *__restrict__ r, int N, const double *__restrict__ q,
+    const int *__restrict__ gfirst, const int *__restrict__ gns, const int *__restrict__ ginfo,
+    double4 *__restrict__ gpos, float4 *__restrict__ spk, int2 *__restrict__ gmeta) {
 #pragma clang fp contract(off)
   const int s = blockIdx.y;
   if (!rebuild[s]) return;
@@ -169,6 +171,22 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
       cnt = nbcap;
     }
     nnb[gs] = cnt;
+  }
+  // the pair kernel's packed inputs in the new sorted order (as k_pack)
+  const int s0 = gfirst[g], ns = gns[g];
+  const double x0 = r[s0], y0 = r[N + s0], z0 = r[2 * N + s0];
+  if (lane == 0) {
+    gpos[gs] = make_double4(x0, y0, z0, 0.0);
+    gmeta[gs] = make_int2(ns, ginfo[g]);
+  }
+  if (lane < MAXS) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane < ns) {
+      const int sk = s0 + lane;
+      v = make_float4((float)(r[sk] - x0), (float)(r[N + sk] - y0), (float)(r[2 * N + sk] - z0),
+                      (float)q[sk]);
+    }
+    spk[(long)gs * MAXS + lane] = v;
   }
 }
 
@@ -243,7 +261,8 @@ int launch_nl_build(Engine &e, int force) {
                      e.d_sp, e.d_rebuild, e.d_cell_start, e.d_order);
   hipLaunchKernelGGL(k_nl_pairs, dim3((e.maxg + 3) / 4, e.nsys), dim3(256), 0, e.stream, e.d_sp,
                      e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start, e.d_last, e.d_sten,
-                     e.nbcap, e.d_nbr, e.d_nnb, e.d_err);
+                     e.nbcap, e.d_nbr, e.d_nnb, e.d_err, e.d_r, e.nsite, e.d_q, e.d_gfirst,
+                     e.d_gns, e.d_ginfo, e.d_gpos, e.d_spk, e.d_gmeta);
   MM_HIP(hipGetLastError());
   return 0;
 }

@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(FFT_TB) k_fft_pass(const SysParams *__restrict
   if (pass == 0 || pass == 4) { nlines = nx * ny; n = nz; twoff = 2 * FFT_MAXN; }
   else { nlines = nx * nzh; n = ny; twoff = FFT_MAXN; }
   if ((int)(blockIdx.x * FFT_LINES) >= nlines) return;
-  load_tw(stw, twid_all + (long)s * 3 * FFT_MAXN + twoff, n);
+  load_tw(stw, twid_all + P.twid0 + twoff, n);
   const int l = threadIdx.x / FFT_T, tid = threadIdx.x % FFT_T;
   const int line = blockIdx.x * FFT_LINES + l;
   const bool on = line < nlines;
@@ -401,7 +401,7 @@ __global__ void __launch_bounds__(FFT_TB) k_fft_xgreen(const SysParams *__restri
   const int nlines = ny * nzh;
   const int nblk = (nlines + FFT_LINES - 1) / FFT_LINES;
   if ((int)blockIdx.x >= nblk) return;
-  load_tw(stw, twid_all + (long)s * 3 * FFT_MAXN, nx);
+  load_tw(stw, twid_all + P.twid0, nx);
   const int l = threadIdx.x / FFT_T, tid = threadIdx.x % FFT_T;
   const int line = blockIdx.x * FFT_LINES + l;
   const bool on = line < nlines;

@@ -52,6 +52,15 @@ __global__ void k_pack(const SysParams *__restrict__ sp, const int *__restrict__
 
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 16); }
 
+// Workgroups are dealt to the 8 XCDs round robin (block b on XCD b % 8).
+// Renumber them so each XCD takes one contiguous eighth of the cell-sorted
+// groups: its L2 then holds the positions of one slab of the box and its
+// halo instead of all of them.  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int per = nb >> 3, rem = nb & 7, x = b & 7, idx = b >> 3;
+  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
 constexpr int NSI = 3;       // sites per group handled by the pair kernel
 constexpr int LANES = 16;    // lanes per i-group
 constexpr int PAIR_TB = 256;
@@ -86,7 +95,8 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
   __shared__ float2 qs[GPB][QCAP];    // switching derivative, neighbour (int bits)
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
-  if ((int)(blockIdx.x * GPB) >= P.ngroup) return;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  if (blk * GPB >= P.ngroup) return;
   const int ntab = P.ntabA + P.ntabB;
   for (int t = threadIdx.x; t < ntab; t += PAIR_TB) stab[t] = tab[P.tab0 + t];
   for (int t = threadIdx.x; t < P.ntype * P.ntype; t += PAIR_TB) slj[t] = ljt[P.lj0 + t];
@@ -95,7 +105,7 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
   const int lane = threadIdx.x & (LANES - 1);
   const int segw = (threadIdx.x & 63) / LANES;     // segment within the wavefront
   const int seg = threadIdx.x / LANES;             // segment within the block
-  const int i = blockIdx.x * GPB + seg;
+  const int i = blk * GPB + seg;
   const bool active = i < P.ngroup;
   const int gs = P.group0 + (active ? i : 0);
 

@@ -1,0 +1,51 @@
+"""Batching: every system of a mixed engine (Ewald and P3M systems of
+different sizes, meshes and assignment orders, in an order that puts Ewald
+systems before P3M ones) gets exactly the forces and energies it gets in an
+engine of its own.  All per-system reductions are deterministic and
+independent of the batch, so the bar is bit equality."""
+import numpy as np
+import pytest
+
+from molmodel_amd import waterbox as W
+from molmodel_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def members():
+    out = []
+    s, p = W.config("water648_ewald")
+    out.append((s, p))
+    s = W.water_box(8, 24.86, 31)
+    p = dict(rc=10.0, smoothing_width=1.0, eta=0.33, kmode=1, mesh=(24, 24, 24), order=7,
+             skin=W.default_skin(10.0, 24.86), nalias=2, ik=0, include_lj_correction=1)
+    out.append((s, p))
+    s, p = W.config("water1000_ions")
+    out.append((s, p))
+    s = W.water_box(10, 31.07, 32)
+    p = dict(rc=10.0, smoothing_width=1.0, eta=0.32, kmode=1, mesh=(30, 30, 30), order=5,
+             skin=W.default_skin(10.0, 31.07), nalias=2, ik=0, include_lj_correction=1)
+    out.append((s, p))
+    return out
+
+
+def test_batch_equals_single_system_engines(gpu_lib):
+    mem = members()
+    systems = [m[0] for m in mem]
+    params = [m[1] for m in mem]
+    e = Engine(systems, params)
+    e.set_positions(np.concatenate([s.r for s in systems]))
+    e.compute()
+    fb = e.forces()
+    eb = e.energies()
+    off = e.offsets
+    for k, (s, p) in enumerate(mem):
+        one = Engine(s, p)
+        one.set_positions(s.r)
+        one.compute()
+        f1, e1 = one.forces(), one.energies()[0]
+        same_f = np.array_equal(fb[off[k]:off[k + 1]], f1)
+        print("system", k, "kmode", p["kmode"], "forces equal", same_f,
+              "max diff", np.abs(fb[off[k]:off[k + 1]] - f1).max(), "energies", eb[k][:4], e1[:4])
+        assert same_f
+        assert np.array_equal(eb[k][:4], e1[:4])
