@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
     int N, int want_virial, double bond, double angle_rad, double tol,
     double *__restrict__ part, int *__restrict__ done, int maxblk, double *__restrict__ acc,
     int *__restrict__ rebuild, int *__restrict__ err, double4 *__restrict__ gpos,
-    float4 *__restrict__ spk) {
+    float4 *__restrict__ spk, const int *__restrict__ ginfo) {
   constexpr int NPART = FinSlots<VIR>::N, TOP = FinSlots<VIR>::TOP;
   __shared__ double red[FIN_TB / 64][NPART];
   __shared__ int islast;
@@ -170,7 +170,8 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
           float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
           if (k < K && k < ns)
             o = make_float4((float)(x[k][0] - x[0][0]), (float)(x[k][1] - x[0][1]),
-                            (float)(x[k][2] - x[0][2]), (float)qq[k]);
+                            (float)(x[k][2] - x[0][2]),
+                            ((ginfo[g] >> (8 * k)) & SITE_HAS_Q) ? (float)qq[k] : 0.f);
           spk[(long)gs * MAXS + k] = o;
         }
       }
@@ -286,7 +287,7 @@ int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out) {
 #define FIN_ARGS e.d_sp, e.d_order, e.d_gfirst, e.d_gns, e.d_gexcl, e.d_q, e.d_mass, e.d_pf, \
     e.d_gacc, e.d_kres, e.d_r, e.d_v, e.d_f, phi_out, e.d_last, e.nsite, want_virial, e.bond,  \
     e.angle * (M_PI / 180.0), e.tol, e.d_part, e.d_done, e.nblk_part, e.d_acc, e.d_rebuild,   \
-    e.d_err, e.d_gpos, e.d_spk
+    e.d_err, e.d_gpos, e.d_spk, e.d_ginfo
 #define FIN_CASE(M)                                                                            \
   if (want_virial)                                                                             \
     hipLaunchKernelGGL((k_finish<M, true>), grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS);         \

@@ -45,21 +45,27 @@ __global__ void __launch_bounds__(256) k_ks_sf(const SysParams *__restrict__ sp,
   }
 }
 
-__global__ void __launch_bounds__(256) k_ks_field(const SysParams *__restrict__ sp,
-                                                  const double4 *__restrict__ kvec,
-                                                  const double *__restrict__ ksc,
-                                                  const double *__restrict__ r, int N,
-                                                  double *__restrict__ kres, double *__restrict__ acc,
-                                                  int want_virial) {
+// Field kernel: a block serves KS_SITES sites; its KS_SPLIT waves split the
+// k-vectors (wave w takes k = w, w + KS_SPLIT, ...), and the per-wave sums
+// are added in wave order (deterministic).
+constexpr int KS_SITES = 64, KS_SPLIT = 16, KS_TB = KS_SITES * KS_SPLIT;
+
+__global__ void __launch_bounds__(KS_TB) k_ks_field(const SysParams *__restrict__ sp,
+                                                    const double4 *__restrict__ kvec,
+                                                    const double *__restrict__ ksc,
+                                                    const double *__restrict__ r, int N,
+                                                    double *__restrict__ kres,
+                                                    double *__restrict__ acc, int want_virial) {
+  __shared__ double part[KS_SPLIT][4][KS_SITES];
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
   if (P.kmode != 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((int)(blockIdx.x * KS_SITES) >= P.nsite) return;
   if (blockIdx.x == 0) {
     // k-sum energy and virial of this system
     double e = 0, w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const double c4 = 0.25 / (P.eta * P.eta);
-    for (int k = threadIdx.x; k < P.nk; k += blockDim.x) {
+    for (int k = threadIdx.x; k < P.nk; k += KS_TB) {
       const double4 kv = kvec[P.k0 + k];
       const double S = ksc[2 * (P.k0 + k)], C = ksc[2 * (P.k0 + k) + 1];
       const double ek = kv.w * (S * S + C * C);
@@ -72,7 +78,7 @@ __global__ void __launch_bounds__(256) k_ks_field(const SysParams *__restrict__ 
           for (int b = 0; b < 3; b++) w[3 * a + b] += ek * ((a == b ? 1.0 : 0.0) - f * kk[a] * kk[b]);
       }
     }
-    __shared__ double red[4][10];
+    __shared__ double red[KS_TB / 64][10];
     for (int o = 32; o >= 1; o >>= 1) {
       e += __shfl_xor(e, o);
       for (int c = 0; c < 9; c++) w[c] += __shfl_xor(w[c], o);
@@ -85,18 +91,20 @@ __global__ void __launch_bounds__(256) k_ks_field(const SysParams *__restrict__ 
     __syncthreads();
     if (threadIdx.x == 0) {
       double t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-      for (int ww = 0; ww < (int)(blockDim.x >> 6); ww++)
+      for (int ww = 0; ww < KS_TB / 64; ww++)
         for (int c = 0; c < 10; c++) t[c] += red[ww][c];
       acc[s * NACC + ACC_KSPACE_E] = t[0];
       // reciprocal virial kept beside the per-site results; finish adds it
       for (int c = 0; c < 9; c++) kres[(long)4 * N + 9 * s + c] = want_virial ? t[1 + c] : 0.0;
     }
   }
-  if (i >= P.nsite) return;
-  const int si = P.site0 + i;
+  const int lane = threadIdx.x & (KS_SITES - 1), split = threadIdx.x / KS_SITES;
+  const int i = blockIdx.x * KS_SITES + lane;
+  const bool on = i < P.nsite;
+  const int si = P.site0 + (on ? i : 0);
   const double x = r[si], y = r[N + si], z = r[2 * N + si];
   double p = 0, gx = 0, gy = 0, gz = 0;
-  for (int k = 0; k < P.nk; k++) {
+  for (int k = split; k < P.nk; k += KS_SPLIT) {
     const double4 kv = kvec[P.k0 + k];
     const double S = ksc[2 * (P.k0 + k)], C = ksc[2 * (P.k0 + k) + 1];
     double sn, cs;
@@ -106,10 +114,16 @@ __global__ void __launch_bounds__(256) k_ks_field(const SysParams *__restrict__ 
     gx += t * kv.x; gy += t * kv.y; gz += t * kv.z;
     p += two_a * (S * sn + C * cs);
   }
-  kres[4 * si] = p;
-  kres[4 * si + 1] = gx;
-  kres[4 * si + 2] = gy;
-  kres[4 * si + 3] = gz;
+  part[split][0][lane] = p;
+  part[split][1][lane] = gx;
+  part[split][2][lane] = gy;
+  part[split][3][lane] = gz;
+  __syncthreads();
+  if (split < 4 && on) {   // four waves finish one component each
+    double t = 0;
+    for (int w = 0; w < KS_SPLIT; w++) t += part[w][split][lane];
+    kres[4 * si + split] = t;
+  }
 }
 
 int launch_kspace(Engine &e, int want_virial, hipStream_t st) {
@@ -119,7 +133,7 @@ int launch_kspace(Engine &e, int want_virial, hipStream_t st) {
   if (maxk == 0) return 0;
   hipLaunchKernelGGL(k_ks_sf, dim3(maxk, e.nsys), dim3(256), 0, st, e.d_sp,
                      (const double4 *)e.d_kvec, e.d_r, e.d_q, e.nsite, e.d_ksc);
-  hipLaunchKernelGGL(k_ks_field, dim3((maxn + 255) / 256, e.nsys), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_ks_field, dim3((maxn + KS_SITES - 1) / KS_SITES, e.nsys), dim3(KS_TB), 0, st,
                      e.d_sp, (const double4 *)e.d_kvec, e.d_ksc, e.d_r, e.nsite, e.d_kres,
                      e.d_acc, want_virial);
   MM_HIP(hipGetLastError());

@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
     if (lane < ns) {
       const int sk = s0 + lane;
       v = make_float4((float)(r[sk] - x0), (float)(r[N + sk] - y0), (float)(r[2 * N + sk] - z0),
-                      (float)q[sk]);
+                      ((ginfo[g] >> (8 * lane)) & SITE_HAS_Q) ? (float)q[sk] : 0.f);
     }
     spk[(long)gs * MAXS + lane] = v;
   }

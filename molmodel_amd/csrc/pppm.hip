@@ -180,45 +180,68 @@ __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict_
   for (int t = threadIdx.x; t < npts; t += blockDim.x) acc[t] = 0ull;
   __syncthreads();
   constexpr int istart = -((P + 1) / 2 - 1);
-  // neighbouring column blocks whose sites reach the owned columns
+  // neighbouring column blocks whose sites reach the owned columns (at most
+  // 8 per axis for P <= 7), flattened into one list of sites so that all
+  // threads work whatever the bins' sizes: thread t < nbins takes bin t,
+  // wave 0 scans the counts
+  __shared__ int sb_start[64], sb_off[65];
   int xlo = floor_div(X0 - istart - P + 1, bx), xhi = floor_div(X0 + bx - 1 - istart, bx);
   int ylo = floor_div(Y0 - istart - P + 1, by), yhi = floor_div(Y0 + by - 1 - istart, by);
   if (xhi - xlo + 1 >= Q.nbx) { xlo = 0; xhi = Q.nbx - 1; }
   if (yhi - ylo + 1 >= Q.nby) { ylo = 0; yhi = Q.nby - 1; }
-  for (int cx = xlo; cx <= xhi; cx++) {
-    const int ccx = mod_nonneg(cx, Q.nbx);
-    for (int cy = ylo; cy <= yhi; cy++) {
-      const int ccy = mod_nonneg(cy, Q.nby);
+  const int ncy = yhi - ylo + 1, nbin = (xhi - xlo + 1) * ncy;   // <= 64
+  if (threadIdx.x < 64) {
+    int cnt = 0;
+    const int t = threadIdx.x;
+    if (t < nbin) {
+      const int ccx = mod_nonneg(xlo + t / ncy, Q.nbx), ccy = mod_nonneg(ylo + t % ncy, Q.nby);
       const int b = Q.bin0 + ccx * Q.nby + ccy;
-      const int b0 = bin_start[b], b1 = bin_start[b + 1];
-      for (int t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
-        const int4 g = bgrid[t];
-        const float4 f = bfrac[t];
-        float wx[P], wy[P], wz[P];
-        bspline_w<P>(f.x, wx, nullptr);
-        bspline_w<P>(f.y, wy, nullptr);
-        bspline_w<P>(f.z, wz, nullptr);
-        int lx[P], ly[P], iz[P];
+      sb_start[t] = bin_start[b];
+      cnt = bin_start[b + 1] - sb_start[t];
+    }
+    int incl = cnt;   // inclusive scan over the wave
 #pragma unroll
-        for (int a = 0; a < P; a++) {
-          lx[a] = wrap(g.x + istart + a, nx) - X0;
-          ly[a] = wrap(g.y + istart + a, ny) - Y0;
-          iz[a] = wrap(g.z + istart + a, nz);
-        }
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o);
+      if (t >= o) incl += y;
+    }
+    sb_off[t + 1] = incl;
+    if (t == 0) sb_off[0] = 0;
+  }
+  __syncthreads();
+  const int total = sb_off[nbin];
+  for (int u = threadIdx.x; u < total; u += blockDim.x) {
+    int lo = 0, hi = nbin - 1;   // last bin with sb_off[bin] <= u
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (sb_off[mid] <= u) lo = mid; else hi = mid - 1;
+    }
+    const int t = sb_start[lo] + (u - sb_off[lo]);
+    const int4 g = bgrid[t];
+    const float4 f = bfrac[t];
+    float wx[P], wy[P], wz[P];
+    bspline_w<P>(f.x, wx, nullptr);
+    bspline_w<P>(f.y, wy, nullptr);
+    bspline_w<P>(f.z, wz, nullptr);
+    int lx[P], ly[P], iz[P];
 #pragma unroll
-        for (int a = 0; a < P; a++) {
-          if (lx[a] < 0 || lx[a] >= bx) continue;
-          const float qa = f.w * wx[a];
+    for (int a = 0; a < P; a++) {
+      lx[a] = wrap(g.x + istart + a, nx) - X0;
+      ly[a] = wrap(g.y + istart + a, ny) - Y0;
+      iz[a] = wrap(g.z + istart + a, nz);
+    }
 #pragma unroll
-          for (int bb = 0; bb < P; bb++) {
-            if (ly[bb] < 0 || ly[bb] >= by) continue;
-            const float qw = qa * wy[bb];
-            unsigned long long *col = acc + (lx[a] * by + ly[bb]) * nz;
+    for (int a = 0; a < P; a++) {
+      if (lx[a] < 0 || lx[a] >= bx) continue;
+      const float qa = f.w * wx[a];
 #pragma unroll
-            for (int c = 0; c < P; c++)
-              atomicAdd(col + iz[c], (unsigned long long)__float2ll_rn(qw * wz[c] * FIX_SCALE));
-          }
-        }
+      for (int bb = 0; bb < P; bb++) {
+        if (ly[bb] < 0 || ly[bb] >= by) continue;
+        const float qw = qa * wy[bb];
+        unsigned long long *col = acc + (lx[a] * by + ly[bb]) * nz;
+#pragma unroll
+        for (int c = 0; c < P; c++)
+          atomicAdd(col + iz[c], (unsigned long long)__float2ll_rn(qw * wz[c] * FIX_SCALE));
       }
     }
   }
