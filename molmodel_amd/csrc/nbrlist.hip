@@ -113,9 +113,12 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
   const int s = blockIdx.y;
   if (!rebuild[s]) return;
   const SysParams &P = sp[s];
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (wave >= P.ngroup) return;
+  // grid-stride over sorted groups: a small grid keeps the launch cheap on
+  // the many steps that do not rebuild
+  const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
+  for (int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); wave < P.ngroup;
+       wave += nwave) {
   const int gs = P.group0 + wave;
   const int g = order[gs];
   const int n = P.ncell;
@@ -188,6 +191,7 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
     }
     spk[(long)gs * MAXS + lane] = v;
   }
+  }
 }
 
 __global__ void k_set_flags(int *f, int n, int v) {
@@ -259,7 +263,8 @@ int launch_nl_build(Engine &e, int force) {
   for (auto &p : e.hsp) maxc = std::max(maxc, p.ncell * p.ncell * p.ncell);
   hipLaunchKernelGGL(k_nl_sortcell, dim3((maxc + tb - 1) / tb, e.nsys), dim3(tb), 0, e.stream,
                      e.d_sp, e.d_rebuild, e.d_cell_start, e.d_order);
-  hipLaunchKernelGGL(k_nl_pairs, dim3((e.maxg + 3) / 4, e.nsys), dim3(256), 0, e.stream, e.d_sp,
+  hipLaunchKernelGGL(k_nl_pairs, dim3(std::min((e.maxg + 3) / 4, 1024), e.nsys), dim3(256), 0,
+                     e.stream, e.d_sp,
                      e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start, e.d_last, e.d_sten,
                      e.nbcap, e.d_nbr, e.d_nnb, e.d_err, e.d_r, e.nsite, e.d_q, e.d_gfirst,
                      e.d_gns, e.d_ginfo, e.d_gpos, e.d_spk, e.d_gmeta);

@@ -680,7 +680,11 @@ extern "C" void p3m_scale(p3m_t p, double s) {   // pppm.c:643-664
   p3m_upload_params(p);
 }
 
-extern "C" p3m_t p3m_copy(const p3m_t t, int nsite) {   // pppm.c:294-311
+// pppm.c:294-311.  The reference memcpy's n values' worth of bytes over the
+// arrays' row-pointer tables (its kvec/green_hat are pointer-of-pointer
+// arrays); what it means is a copy with the same influence function, which
+// is what this does.
+extern "C" p3m_t p3m_copy(const p3m_t t, int nsite) {
   int ng[3];
   p3m_t p = p3m_new(nsite, t->latv, t->grid_spacing, t->nalias, t->ik, t->order, ng);
   p->green = t->green;

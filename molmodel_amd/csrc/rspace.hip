@@ -188,100 +188,101 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
       __builtin_amdgcn_wave_barrier();
     }
     if (!__any(qn > 0)) break;
-    // ---- compute one batch per segment.  The body is written without
-    // data-dependent branches around the accumulators: lanes past the batch
-    // evaluate a dummy pair (own group, 5 A away, switching value 0) whose
-    // every contribution is an exact zero, absent sites carry charge 0 and
-    // no LJ bit, and only the LJ term and the rare out-of-table branch are
-    // conditional.
+    // ---- compute one batch per segment
     const int nb = qn < LANES ? qn : LANES;
-    const bool live = lane < nb;
-    const int qsl = qn - nb + (live ? lane : 0);
-    const float4 entq = myqd[qsl];
-    const float2 e2q = myqs[qsl];
-    const int hs = live ? __float_as_int(e2q.y) : gs;
-    const float smf = live ? entq.w : 0.f, dsmf = live ? e2q.x : 0.f;
-    const float Dx = live ? entq.x : 5.f, Dy = live ? entq.y : 0.f, Dz = live ? entq.z : 0.f;
-    const int2 mj = gmeta[hs];
-    float fx[NSI], fy[NSI], fz[NSI], ph[NSI], gx[NSI], gy[NSI], gz[NSI];
+    if (lane < nb) {
+      const float4 ent = myqd[qn - nb + lane];
+      const float2 e2 = myqs[qn - nb + lane];
+      const int hs = __float_as_int(e2.y);
+      const float smf = ent.w, dsmf = e2.x;
+      const int2 mj = gmeta[hs];
+      const int nsj = mj.x;
+      const float Dx = ent.x, Dy = ent.y, Dz = ent.z;
+      float fx[NSI], fy[NSI], fz[NSI], ph[NSI], gx[NSI], gy[NSI], gz[NSI];
 #pragma unroll
-    for (int k = 0; k < NSI; k++) { fx[k] = fy[k] = fz[k] = ph[k] = 0.f; gx[k] = gy[k] = gz[k] = 0.f; }
-    float tmps = 0.f, uu = 0.f;
+      for (int k = 0; k < NSI; k++) { fx[k] = fy[k] = fz[k] = ph[k] = 0.f; gx[k] = gy[k] = gz[k] = 0.f; }
+      float tmps = 0.f, uu = 0.f;
 #pragma unroll
-    for (int l = 0; l < NSI; l++) {
-      const float4 oj = spk[(long)hs * MAXS + l];   // zeros past the group's sites
-      const int bj = (mj.y >> (8 * l)) & 0xff;
-      const int mkj = bj & 3, tyj = bj >> 2;
-      const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
-      const float qj = oj.w;                        // 0 for sites without charge
-      const float qs = qj * smf;                    // smoothed charge of site l
-      const float qsA = qs * m2A, qsB = qs * m2B;
-      float tq = 0.f;                               // sum_k q_k b0_k
+      for (int l = 0; l < NSI; l++) {
+        if (l >= nsj) break;
+        const float4 oj = spk[(long)hs * MAXS + l];
+        const int bj = (mj.y >> (8 * l)) & 0xff;
+        const int mkj = bj & 3, tyj = bj >> 2;
+        const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
+        const float qj = oj.w;
+        const float qs = qj * smf;        // smoothed charge of site l
+        const float qsA = qs * m2A, qsB = qs * m2B;
+        float tq = 0.f;                   // sum_k q_k b0_k (unsmoothed energy / q_l)
+#pragma unroll
+        for (int k = 0; k < NSI; k++) {
+          const int m = mki[k] & mkj;
+          if (!m) continue;
+          const float rx = px + oi[k].x, ry = py + oi[k].y, rz = pz + oi[k].z;
+          const float r2 = rx * rx + ry * ry + rz * rz;
+          if (m & SITE_HAS_Q) {
+            float b0, e;
+            if (r2 >= tAmin && r2 < tBmax) {
+              const float xa = fmaf(r2, tAinv, offA), xb = fmaf(r2, tBinv, offB);
+              const bool inA = xa < xb;
+              const float xx = inA ? xa : xb;
+              const int ii = (int)xx;
+              const float fr = xx - (float)ii;
+              const float4 c = stab[ii];
+              const float p2 = fmaf(c.w, fr, c.z), p1 = fmaf(p2, fr, c.y);
+              b0 = fmaf(p1, fr, c.x);
+              // b1 = -2/dx * d(cubic)/dt (the derivative from the Horner
+              // intermediates), times the smoothed charge
+              e = fmaf(fr, fmaf(c.w, fr, p2), p1) * (inA ? qsA : qsB);
+            } else {                            // rspace.c:120-125, outside the table
+              const float rm = sqrtf(r2), er = eta * rm;
+              b0 = erfcf(er) / rm;
+              e = qs * (b0 + eta * 1.1283791670955126f * expf(-er * er)) / r2;
+            }
+            tq += oi[k].w * b0;
+            ph[k] += qs * b0;
+            fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
+          }
+          if (m & SITE_HAS_LJ) {
+            const float2 c = slj[tyi[k] * nty + tyj];
+            const float ir2 = __builtin_amdgcn_rcpf(r2);
+            const float ir6 = ir2 * ir2 * ir2;
+            const float bi6 = c.x * ir6, ai12 = c.y * ir6 * ir6;
+            const float u = ai12 - bi6;
+            const float du = (12.f * ai12 - 6.f * bi6) * ir2 * smf;
+            uu += u;
+            gx[k] += du * rx; gy[k] += du * ry; gz[k] += du * rz;
+          }
+        }
+        tmps += qj * tq;
+      }
+      tmps += uu;
+      uu *= smf;
+      // smoothing force on the first site (rspace.c:479-489)
+      const float fs = dsmf * tmps;
+      const float sx = fs * Dx, sy = fs * Dy, sz = fs * Dz;
+      double Fx = sx, Fy = sy, Fz = sz;
 #pragma unroll
       for (int k = 0; k < NSI; k++) {
-        const float rx = px + oi[k].x, ry = py + oi[k].y, rz = pz + oi[k].z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        float b0, e;
-        if (r2 >= tAmin && r2 < tBmax) {
-          const float xa = fmaf(r2, tAinv, offA), xb = fmaf(r2, tBinv, offB);
-          const bool inA = xa < xb;
-          const float xx = inA ? xa : xb;
-          const int ii = (int)xx;
-          const float fr = xx - (float)ii;
-          const float4 c = stab[ii];
-          const float p2 = fmaf(c.w, fr, c.z), p1 = fmaf(p2, fr, c.y);
-          b0 = fmaf(p1, fr, c.x);
-          // b1 = -2/dx * d(cubic)/dt (the derivative from the Horner
-          // intermediates), times the smoothed charge
-          e = fmaf(fr, fmaf(c.w, fr, p2), p1) * (inA ? qsA : qsB);
-        } else {                            // rspace.c:120-125, outside the table
-          const float rm = sqrtf(r2), er = eta * rm;
-          b0 = erfcf(er) / rm;
-          e = qs * (b0 + eta * 1.1283791670955126f * expf(-er * er)) / r2;
-        }
-        tq += oi[k].w * b0;
-        ph[k] += qs * b0;
-        fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
-        if (mki[k] & mkj & SITE_HAS_LJ) {
-          const float2 c = slj[tyi[k] * nty + tyj];
-          const float ir2 = __builtin_amdgcn_rcpf(r2);
-          const float ir6 = ir2 * ir2 * ir2;
-          const float bi6 = c.x * ir6, ai12 = c.y * ir6 * ir6;
-          const float u = ai12 - bi6;
-          const float du = (12.f * ai12 - 6.f * bi6) * ir2 * smf;
-          uu += u;
-          gx[k] += du * rx; gy[k] += du * ry; gz[k] += du * rz;
+        ap[k] += (double)ph[k];
+        if (FIELD) {
+          ax[k] += (double)fx[k]; ay[k] += (double)fy[k]; az[k] += (double)fz[k];
+          lx[k] += (double)gx[k]; ly[k] += (double)gy[k]; lz[k] += (double)gz[k];
+        } else {
+          const float qk = oi[k].w;
+          const float tx = qk * fx[k] + gx[k], ty = qk * fy[k] + gy[k], tz = qk * fz[k] + gz[k];
+          ax[k] += (double)tx; ay[k] += (double)ty; az[k] += (double)tz;
+          if (VIRIAL) { Fx += tx; Fy += ty; Fz += tz; }
         }
       }
-      tmps += qj * tq;
-    }
-    tmps += uu;
-    uu *= smf;
-    // smoothing force on the first site (rspace.c:479-489)
-    const float fs = dsmf * tmps;
-    const float sx = fs * Dx, sy = fs * Dy, sz = fs * Dz;
-    double Fx = sx, Fy = sy, Fz = sz;
-#pragma unroll
-    for (int k = 0; k < NSI; k++) {
-      ap[k] += (double)ph[k];
-      if (FIELD) {
-        ax[k] += (double)fx[k]; ay[k] += (double)fy[k]; az[k] += (double)fz[k];
-        lx[k] += (double)gx[k]; ly[k] += (double)gy[k]; lz[k] += (double)gz[k];
-      } else {
-        const float qk = oi[k].w;
-        const float tx = qk * fx[k] + gx[k], ty = qk * fy[k] + gy[k], tz = qk * fz[k] + gz[k];
-        ax[k] += (double)tx; ay[k] += (double)ty; az[k] += (double)tz;
-        if (VIRIAL) { Fx += tx; Fy += ty; Fz += tz; }
+      if (FIELD) { lx[0] += (double)sx; ly[0] += (double)sy; lz[0] += (double)sz; }
+      else { ax[0] += (double)sx; ay[0] += (double)sy; az[0] += (double)sz; }
+      ulj += (double)uu;
+      if (VIRIAL) {   // 1/2 D (x) F_gh per directed pair
+        const double dx = Dx, dy = Dy, dz = Dz;
+        w[0] += 0.5 * dx * Fx; w[1] += 0.5 * dx * Fy; w[2] += 0.5 * dx * Fz;
+        w[3] += 0.5 * dy * Fx; w[4] += 0.5 * dy * Fy; w[5] += 0.5 * dy * Fz;
+        w[6] += 0.5 * dz * Fx; w[7] += 0.5 * dz * Fy; w[8] += 0.5 * dz * Fz;
       }
-    }
-    if (FIELD) { lx[0] += (double)sx; ly[0] += (double)sy; lz[0] += (double)sz; }
-    else { ax[0] += (double)sx; ay[0] += (double)sy; az[0] += (double)sz; }
-    ulj += (double)uu;
-    if (VIRIAL) {   // 1/2 D (x) F_gh per directed pair
-      const double dx = Dx, dy = Dy, dz = Dz;
-      w[0] += 0.5 * dx * Fx; w[1] += 0.5 * dx * Fy; w[2] += 0.5 * dx * Fz;
-      w[3] += 0.5 * dy * Fx; w[4] += 0.5 * dy * Fy; w[5] += 0.5 * dy * Fz;
-      w[6] += 0.5 * dz * Fx; w[7] += 0.5 * dz * Fy; w[8] += 0.5 * dz * Fz;
     }
     qn -= nb;
     __builtin_amdgcn_wave_barrier();
@@ -307,12 +308,8 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
   if (!active || lane != 0) return;
 #pragma unroll
   for (int k = 0; k < NSI; k++) {
-    // sites without a charge get no real-space potential or field
-    // (rspace.c calls the Coulomb kernels only for SITE_HAS_Q pairs)
-    const double qm = (mki[k] & SITE_HAS_Q) ? 1.0 : 0.0;
     double *o = pf + ((long)gs * MAXS + k) * 4;
-    o[0] = FIELD ? ax[k] * qm : ax[k]; o[1] = FIELD ? ay[k] * qm : ay[k];
-    o[2] = FIELD ? az[k] * qm : az[k]; o[3] = ap[k] * qm;
+    o[0] = ax[k]; o[1] = ay[k]; o[2] = az[k]; o[3] = ap[k];
     if (FIELD) {
       double *l = plj + ((long)gs * MAXS + k) * 3;
       l[0] = lx[k]; l[1] = ly[k]; l[2] = lz[k];
