@@ -197,8 +197,9 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
     return 0;
   }
   // The reciprocal-space branch reads only positions and charges and writes
-  // its own buffers, so it runs on the second stream beside the neighbour
-  // list and pair kernels; the join comes before anything that consumes it.
+  // its own buffers, so it runs on the second stream beside the
+  // neighbour-list and pair kernels; the join comes before anything that
+  // consumes it.
   const bool fork = e.stream2 != nullptr;
   hipStream_t kst = e.stream;
   if (fork) {
@@ -208,7 +209,6 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
   }
   TRY(launch_kspace(e, want_virial, kst));
   TRY(launch_p3m(e, want_virial, kst));
-  if (fork) MM_HIP(hipEventRecord(e.ev_join, e.stream2));
   if (e.external_nl) {
     e.first_eval = 0;
   } else if (e.first_eval) {
@@ -222,7 +222,10 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
   e.need_pack = 0;
   m->pos_dirty = 0;
   TRY(launch_pair(e, field_mode, want_virial));
-  if (fork) MM_HIP(hipStreamWaitEvent(e.stream, e.ev_join, 0));
+  if (fork) {
+    MM_HIP(hipEventRecord(e.ev_join, e.stream2));
+    MM_HIP(hipStreamWaitEvent(e.stream, e.ev_join, 0));
+  }
   return 0;
 }
 

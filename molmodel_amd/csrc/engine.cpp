@@ -474,7 +474,9 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
           const double a = -2 * M_PI * (t % nn[d]) / nn[d];
           twid.push_back(make_float2((float)cos(a), (float)sin(a)));
         }
-      MM_REQ(choose_spread_blocks(S.nx, S.ny, S.nz, 4096, 4096, P.bx, P.by),
+      int want = 2;
+      if (const char *v = getenv("MOLMODEL_AMD_SPREAD_B")) want = std::max(2, atoi(v));
+      MM_REQ(choose_spread_blocks(S.nx, S.ny, S.nz, 4096, 4096, P.bx, P.by, want),
              "mesh dimensions give no valid spreading column blocks");
       P.nbx = S.nx / P.bx;
       P.nby = S.ny / P.by;
@@ -516,6 +518,12 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
     MM_HIP(hipStreamCreateWithFlags(&e.stream2, hipStreamNonBlocking));
   MM_HIP(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
   MM_HIP(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
+  for (auto &P : e.hsp) {
+    P.water = P.ngroup > 0;
+    for (int g = P.group0; g < P.group0 + P.ngroup; g++)
+      if (gns[g] != 3 || (ginfo[g] & 0x030303) != WATER_MASKS) P.water = 0;
+    e.any_water |= P.water;
+  }
   MM_TRY(dalloc(&e.d_sp, e.nsys));
   MM_TRY(dput(e.d_sp, e.hsp));
   MM_TRY(dalloc(&e.d_gfirst, G)); MM_TRY(dput(e.d_gfirst, gfirst));
@@ -566,6 +574,23 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   MM_TRY(dalloc(&e.d_acc, (size_t)e.nsys * NACC));
   MM_TRY(dalloc(&e.d_part, (size_t)e.nsys * e.nblk_part * 16));
   MM_TRY(dalloc(&e.d_done, 4 * (size_t)e.nsys));
+  {
+    // persistent pair grid: MOLMODEL_AMD_PAIR_SLOTS workgroups per CU (the
+    // kernel's occupancy allows 4; the default 3 leaves one slot on every
+    // CU to the reciprocal-space kernels of the other stream)
+    std::vector<int> ts(e.nsys + 1, 0);
+    for (int s = 0; s < e.nsys; s++) ts[s + 1] = ts[s] + pair_tiles(e.hsp[s].ngroup);
+    e.ntile = ts[e.nsys];
+    int dev = 0, ncu = 256;
+    MM_HIP(hipGetDevice(&dev));
+    MM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int slots = 3;
+    if (const char *v = getenv("MOLMODEL_AMD_PAIR_SLOTS")) slots = std::max(1, std::min(8, atoi(v)));
+    const int want = std::min(slots * ncu, (e.ntile + 7) / 8 * 8);
+    e.pair_blocks = std::max(8, want / 8 * 8);
+    MM_TRY(dalloc(&e.d_tile_start, e.nsys + 1));
+    MM_TRY(dput(e.d_tile_start, ts));
+  }
   {
     std::vector<int> cs(e.ncell_total + 1, 0);
     cs[e.ncell_total] = G;
@@ -669,7 +694,7 @@ void engine_destroy(Engine &e) {
                   e.d_sgrid, e.d_sfrac, e.d_bin_count, e.d_bin_start, e.d_bin_fill,
                   e.d_bgrid, e.d_bfrac, e.d_acc, e.d_part, e.d_done, e.d_phi, e.d_scratch,
                   e.d_nlb, e.d_ladder, e.d_kT, e.d_tail, e.d_rtmp, e.d_mom, e.d_perm,
-                  e.d_remd_stats};
+                  e.d_remd_stats, e.d_tile_start};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (e.stream) hipStreamDestroy(e.stream);

@@ -114,6 +114,11 @@ struct Engine {
   double *d_acc = nullptr;    // NACC per system
   double *d_part = nullptr;   // per-block partials
   int *d_done = nullptr;      // per-system block counters
+  // pair-kernel work: tiles of 16 i-groups numbered over all systems
+  int any_water = 0;          // some system holds only three-site waters (SysParams::water)
+  int ntile = 0;
+  int pair_blocks = 0;        // persistent grid (a multiple of 8)
+  int *d_tile_start = nullptr;   // nsys + 1
   int nblk_finish = 0;         // finish blocks per system (max)
   int nblk_part = 0;           // partial-sum slots per system
   // MD
@@ -159,6 +164,7 @@ int engine_scale_system(Engine &e, int s, double f);
 int launch_nl_build(Engine &e, int force);
 int launch_pack(Engine &e);
 int launch_pair(Engine &e, int field_mode, int want_virial);
+int pair_tiles(int ngroup);
 // mesh lines per FFT workgroup (16 threads per line); the x pass writes one
 // partial-sum slot per workgroup
 constexpr int P3M_FFT_LINES = 16;

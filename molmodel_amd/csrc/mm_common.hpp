@@ -20,6 +20,9 @@ namespace mmamd {
 constexpr int MAXS = 4;          // max sites per group handled on the device
 constexpr int SITE_HAS_Q = 1;    // rspace.h:7
 constexpr int SITE_HAS_LJ = 2;   // rspace.h:8
+// mask bytes of a three-site water (charges on all sites, LJ on the first
+// only): Q|LJ, Q, Q
+constexpr int WATER_MASKS = (SITE_HAS_Q | SITE_HAS_LJ) | (SITE_HAS_Q << 8) | (SITE_HAS_Q << 16);
 constexpr int MAXORDER = 7;      // pppm.c:61
 
 // error flags (bitwise OR into Engine::d_err)
@@ -57,6 +60,7 @@ struct SysParams {
   double RL[9];                // reciprocal lattice 2 pi (L^-1)^T, tensor_t layout
   int bx, by, nbx, nby, bin0;  // spread column blocks (bx x by mesh columns each)
   int twid0;                   // first of this system's 3 x 128 FFT twiddles
+  int water;                   // every group is a three-site water (mask bytes WATER_MASKS)
   // dynamics
   double dt;
   double bg_phi;               // neutralising background potential (or 0)
@@ -70,19 +74,21 @@ __host__ __device__ inline int mod_nonneg(int n, int m) {   // fns.h:48-58
   return k > 0 ? m - k : 0;
 }
 
-// Mesh columns per spreading block along x and y: the smallest factor >= 2
-// of each mesh size (2 x 2 columns for even meshes: many small workgroups),
-// falling back to single columns where the block's points (bx by nz, 64-bit
-// each in LDS) would exceed maxpts or the block count would exceed maxbins.
+// Mesh columns per spreading block along x and y: the smallest factor >= want
+// (at most 16) of each mesh size, else the smallest factor >= 2, falling back
+// to single columns where the block's points (bx by nz, 64-bit each in LDS)
+// would exceed maxpts or the block count would exceed maxbins.
 inline bool choose_spread_blocks(int nx, int ny, int nz, int maxpts, int maxbins, int &bx,
-                                 int &by) {
-  auto first_factor = [](int n) {
+                                 int &by, int want = 2) {
+  auto factor = [want](int n) {
+    for (int b = want; b <= 16; b++)
+      if (n % b == 0) return b;
     for (int b = 2; b <= 7; b++)
       if (n % b == 0) return b;
     return 1;
   };
-  bx = first_factor(nx);
-  by = first_factor(ny);
+  bx = factor(nx);
+  by = factor(ny);
   if (bx * by * nz > maxpts) bx = 1;
   if (bx * by * nz > maxpts) by = 1;
   return bx * by * nz <= maxpts && (nx / bx) * (ny / by) <= maxbins;

@@ -337,7 +337,7 @@ __device__ __forceinline__ void load_tw(float2 *stw, const float2 *tw, int n) {
 
 // pass 0: z lines, real -> half spectrum.  pass 1/3: y lines fwd/inv.
 // pass 4: z lines, half spectrum -> real.
-__global__ void __launch_bounds__(FFT_TB) k_fft_pass(const SysParams *__restrict__ sp,
+__global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restrict__ sp,
                                                      const float2 *__restrict__ twid_all,
                                                      float *__restrict__ mesh,
                                                      float2 *__restrict__ cmesh, int pass) {
@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(FFT_TB) k_fft_pass(const SysParams *__restrict
 // Also the mesh energy 1/2 sum G |rho_k|^2 and its virial
 // W_ab = sum e_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 eta^2))).
 template <bool VIR>
-__global__ void __launch_bounds__(FFT_TB) k_fft_xgreen(const SysParams *__restrict__ sp,
+__global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__restrict__ sp,
                                                        const float2 *__restrict__ twid_all,
                                                        const float *__restrict__ green,
                                                        float2 *__restrict__ cmesh,

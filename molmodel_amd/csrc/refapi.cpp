@@ -583,6 +583,8 @@ extern "C" p3m_t p3m_new(int nsite, const double ptv[9], double grid_spacing, in
   P.nsite = nsite;
   P.nx = p->ngrid[0]; P.ny = p->ngrid[1]; P.nz = p->ngrid[2];
   P.order = assignment_order;
+  if (P.nx < P.order || P.ny < P.order || P.nz < P.order)
+    die_msg("p3m_new", "mesh dimensions below the assignment order are not supported on the device");
   const int nzh = P.nz / 2 + 1;
   if (!choose_spread_blocks(P.nx, P.ny, P.nz, 4096, 4096, P.bx, P.by))
     die_msg("p3m_new", "mesh dimensions give no valid spreading column blocks");

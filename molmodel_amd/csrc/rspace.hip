@@ -52,14 +52,6 @@ __global__ void k_pack(const SysParams *__restrict__ sp, const int *__restrict__
 
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 16); }
 
-// Workgroups are dealt to the 8 XCDs round robin (block b on XCD b % 8).
-// Renumber them so each XCD takes one contiguous eighth of the cell-sorted
-// groups: its L2 then holds the positions of one slab of the box and its
-// halo instead of all of them.  A bijection on [0, nb).
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-  const int per = nb >> 3, rem = nb & 7, x = b & 7, idx = b >> 3;
-  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
-}
 
 constexpr int NSI = 3;       // sites per group handled by the pair kernel
 constexpr int LANES = 16;    // lanes per i-group
@@ -82,26 +74,46 @@ constexpr int QCAP = 32;      // queue entries per i-group (one batch of 16 + a 
 //            into FP64 accumulators.
 // Lanes therefore spend the expensive phase almost only on pairs inside the
 // cutoff.
-template <bool FIELD, bool VIRIAL>
-__global__ void __launch_bounds__(PAIR_TB) k_pair(
-    const SysParams *__restrict__ sp, const double4 *__restrict__ gpos,
-    const float4 *__restrict__ spk, const int2 *__restrict__ gmeta,
-    const int *__restrict__ nbr, const int *__restrict__ nnb, int nbcap,
-    const float4 *__restrict__ tab, const float2 *__restrict__ ljt,
-    double *__restrict__ pf, double *__restrict__ plj, double *__restrict__ gacc) {
-  __shared__ float4 stab[MAXTAB];
-  __shared__ float2 slj[MAXLJ];
-  __shared__ float4 qd[GPB][QCAP];    // Dx, Dy, Dz, switching value
-  __shared__ float2 qs[GPB][QCAP];    // switching derivative, neighbour (int bits)
-  const int s = blockIdx.y;
-  const SysParams &P = sp[s];
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
-  if (blk * GPB >= P.ngroup) return;
-  const int ntab = P.ntabA + P.ntabB;
-  for (int t = threadIdx.x; t < ntab; t += PAIR_TB) stab[t] = tab[P.tab0 + t];
-  for (int t = threadIdx.x; t < P.ntype * P.ntype; t += PAIR_TB) slj[t] = ljt[P.lj0 + t];
-  __syncthreads();
+// erfc(eta r)/r of one charged site pair (b0) and its radial force factor
+// times the smoothed charge qs (e), from the LDS table inside its range and
+// from the functions outside it (rspace.c:120-125).  Both table segments
+// share one index space: x_A(r2) = (r2 - Amin) / dxA and x_B(r2) = nA +
+// (r2 - Mid) / dxB meet at r2 = Mid, and A is the steeper map, so the table
+// index is min(x_A, x_B) and the segment is the one giving the minimum.
+struct TabView {
+  float amin, bmax, ainv, binv, offA, offB, m2A, m2B, eta;
+};
 
+__device__ __forceinline__ void coul_pair(const TabView &T, const float4 *stab, float r2, float qs,
+                                          float qsA, float qsB, float &b0, float &e) {
+  if (r2 >= T.amin && r2 < T.bmax) {
+    const float xa = fmaf(r2, T.ainv, T.offA), xb = fmaf(r2, T.binv, T.offB);
+    const bool inA = xa < xb;
+    const float xx = inA ? xa : xb;
+    const int ii = (int)xx;
+    const float fr = xx - (float)ii;
+    const float4 c = stab[ii];
+    const float p2 = fmaf(c.w, fr, c.z), p1 = fmaf(p2, fr, c.y);
+    b0 = fmaf(p1, fr, c.x);
+    // b1 = -2/dx * d(cubic)/dt (the derivative from the Horner
+    // intermediates), times the smoothed charge (qsA = qs * -2/dxA, qsB
+    // likewise)
+    e = fmaf(fr, fmaf(c.w, fr, p2), p1) * (inA ? qsA : qsB);
+  } else {
+    const float rm = sqrtf(r2), er = T.eta * rm;
+    b0 = erfcf(er) / rm;
+    e = qs * (b0 + T.eta * 1.1283791670955126f * expf(-er * er)) / r2;
+  }
+}
+
+// One tile: the GPB i-groups blk * GPB ... of system P.  The block's LDS
+// holds that system's erfc and LJ tables.
+template <bool FIELD, bool VIRIAL, bool WATER>
+__device__ __forceinline__ void pair_tile(
+    const SysParams &P, int blk, const float4 *stab, const float2 *slj, float4 (*qd)[QCAP],
+    float2 (*qs)[QCAP], const double4 *__restrict__ gpos, const float4 *__restrict__ spk,
+    const int2 *__restrict__ gmeta, const int *__restrict__ nbr, const int *__restrict__ nnb,
+    int nbcap, double *__restrict__ pf, double *__restrict__ plj, double *__restrict__ gacc) {
   const int lane = threadIdx.x & (LANES - 1);
   const int segw = (threadIdx.x & 63) / LANES;     // segment within the wavefront
   const int seg = threadIdx.x / LANES;             // segment within the block
@@ -131,16 +143,13 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
   for (int c = 0; c < 9; c++) w[c] = 0;
 
   const double L = P.L, invL = P.invL, rhi2 = P.rhi2, rlo2 = P.rlo2;
-  const float tAmin = P.tabAmin, tMid = P.tabMid, tAinv = P.tabAinv, tBinv = P.tabBinv,
-              tBmax = P.tabBmax;
-  // Both table segments in one index space: x_A(r2) = (r2 - Amin) / dxA and
-  // x_B(r2) = nA + (r2 - Mid) / dxB meet at r2 = Mid, and A is the steeper
-  // map, so the table index is min(x_A, x_B) and the segment is the one
-  // giving the minimum.
-  const float offA = -tAmin * tAinv, offB = (float)P.ntabA - tMid * tBinv;
-  const float m2A = -2.f * tAinv, m2B = -2.f * tBinv;
+  TabView T;
+  T.amin = P.tabAmin; T.bmax = P.tabBmax; T.ainv = P.tabAinv; T.binv = P.tabBinv;
+  T.offA = -P.tabAmin * P.tabAinv;
+  T.offB = (float)P.ntabA - P.tabMid * P.tabBinv;
+  T.m2A = -2.f * P.tabAinv; T.m2B = -2.f * P.tabBinv;
+  T.eta = (float)P.eta;
   const int nty = P.ntype;
-  const float eta = (float)P.eta;
   const int cnt = active ? nnb[gs] : 0;
   const int *row = nbr + (long)gs * nbcap;
   float4 *myqd = qd[seg];
@@ -202,16 +211,51 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
 #pragma unroll
       for (int k = 0; k < NSI; k++) { fx[k] = fy[k] = fz[k] = ph[k] = 0.f; gx[k] = gy[k] = gz[k] = 0.f; }
       float tmps = 0.f, uu = 0.f;
+      if (WATER && P.water) {
+        // every group of this system is a three-site water: nine charged
+        // site pairs and the first sites' LJ pair, without per-site mask
+        // branches (a tile never spans two systems, so the branch is
+        // uniform)
+        {   // first sites: LJ
+          const float4 o0 = spk[(long)hs * MAXS];
+          const float rx = Dx - o0.x + oi[0].x, ry = Dy - o0.y + oi[0].y, rz = Dz - o0.z + oi[0].z;
+          const float r2 = rx * rx + ry * ry + rz * rz;
+          const float2 c = slj[tyi[0] * nty + ((mj.y >> 2) & 0x3f)];
+          const float ir2 = __builtin_amdgcn_rcpf(r2);
+          const float ir6 = ir2 * ir2 * ir2;
+          const float bi6 = c.x * ir6, ai12 = c.y * ir6 * ir6;
+          uu = ai12 - bi6;
+          const float du = (12.f * ai12 - 6.f * bi6) * ir2 * smf;
+          gx[0] = du * rx; gy[0] = du * ry; gz[0] = du * rz;
+        }
+#pragma unroll 1
+        for (int l = 0; l < 3; l++) {
+          const float4 oj = spk[(long)hs * MAXS + l];
+          const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
+          const float qs = oj.w * smf, qsA = qs * T.m2A, qsB = qs * T.m2B;
+          float tq = 0.f;
 #pragma unroll
-      for (int l = 0; l < NSI; l++) {
-        if (l >= nsj) break;
+          for (int k = 0; k < 3; k++) {
+            const float rx = px + oi[k].x, ry = py + oi[k].y, rz = pz + oi[k].z;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            float b0, e;
+            coul_pair(T, stab, r2, qs, qsA, qsB, b0, e);
+            tq += oi[k].w * b0;
+            ph[k] += qs * b0;
+            fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
+          }
+          tmps += oj.w * tq;
+        }
+      } else {
+#pragma unroll 1
+      for (int l = 0; l < nsj; l++) {
         const float4 oj = spk[(long)hs * MAXS + l];
         const int bj = (mj.y >> (8 * l)) & 0xff;
         const int mkj = bj & 3, tyj = bj >> 2;
         const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
         const float qj = oj.w;
         const float qs = qj * smf;        // smoothed charge of site l
-        const float qsA = qs * m2A, qsB = qs * m2B;
+        const float qsA = qs * T.m2A, qsB = qs * T.m2B;
         float tq = 0.f;                   // sum_k q_k b0_k (unsmoothed energy / q_l)
 #pragma unroll
         for (int k = 0; k < NSI; k++) {
@@ -221,23 +265,7 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
           const float r2 = rx * rx + ry * ry + rz * rz;
           if (m & SITE_HAS_Q) {
             float b0, e;
-            if (r2 >= tAmin && r2 < tBmax) {
-              const float xa = fmaf(r2, tAinv, offA), xb = fmaf(r2, tBinv, offB);
-              const bool inA = xa < xb;
-              const float xx = inA ? xa : xb;
-              const int ii = (int)xx;
-              const float fr = xx - (float)ii;
-              const float4 c = stab[ii];
-              const float p2 = fmaf(c.w, fr, c.z), p1 = fmaf(p2, fr, c.y);
-              b0 = fmaf(p1, fr, c.x);
-              // b1 = -2/dx * d(cubic)/dt (the derivative from the Horner
-              // intermediates), times the smoothed charge
-              e = fmaf(fr, fmaf(c.w, fr, p2), p1) * (inA ? qsA : qsB);
-            } else {                            // rspace.c:120-125, outside the table
-              const float rm = sqrtf(r2), er = eta * rm;
-              b0 = erfcf(er) / rm;
-              e = qs * (b0 + eta * 1.1283791670955126f * expf(-er * er)) / r2;
-            }
+            coul_pair(T, stab, r2, qs, qsA, qsB, b0, e);
             tq += oi[k].w * b0;
             ph[k] += qs * b0;
             fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
@@ -254,6 +282,7 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
           }
         }
         tmps += qj * tq;
+      }
       }
       tmps += uu;
       uu *= smf;
@@ -329,6 +358,54 @@ __global__ void __launch_bounds__(PAIR_TB) k_pair(
     }
 #pragma unroll
     for (int c = 0; c < 9; c++) ga[1 + c] = w[c];
+  }
+}
+
+// Persistent pair kernel.  Tiles of GPB i-groups are numbered over all
+// systems (system s owns tiles [tile_start[s], tile_start[s+1])) and split
+// into 8 contiguous ranges, one per class of blocks with the same
+// blockIdx.x mod 8.  Workgroups are dealt to the 8 XCDs round robin, so a
+// class is the set of blocks on one XCD, and that XCD's L2 then holds the
+// positions of one slab of the cell-sorted groups and its halo instead of
+// all of them.  Block k of a class takes tiles lo + k, lo + k + nper, ...;
+// the tiles in flight at any time are thus one contiguous window of the
+// slab.  Waves move on to their next tile without waiting for each other
+// (each wave owns its segments' queues); a block synchronises only to load
+// another system's tables.  The host sizes the grid below the device's
+// resident capacity, which leaves room on every CU for the reciprocal-space
+// kernels running on the other stream.
+template <bool FIELD, bool VIRIAL, bool WATER>
+__global__ void __launch_bounds__(PAIR_TB, (FIELD || VIRIAL) ? 1 : 4) k_pair(
+    const SysParams *__restrict__ sp, const int *__restrict__ tile_start, int nsys, int ntile,
+    const double4 *__restrict__ gpos, const float4 *__restrict__ spk,
+    const int2 *__restrict__ gmeta, const int *__restrict__ nbr, const int *__restrict__ nnb,
+    int nbcap, const float4 *__restrict__ tab, const float2 *__restrict__ ljt,
+    double *__restrict__ pf, double *__restrict__ plj, double *__restrict__ gacc) {
+  __shared__ float4 stab[MAXTAB];
+  __shared__ float2 slj[MAXLJ];
+  __shared__ float4 qd[GPB][QCAP];    // Dx, Dy, Dz, switching value
+  __shared__ float2 qs[GPB][QCAP];    // switching derivative, neighbour (int bits)
+  const int cls = blockIdx.x & 7;
+  const int nper = (int)((gridDim.x + 7 - cls) >> 3);   // blocks in this class
+  const int lo = (int)((long)ntile * cls / 8), hi = (int)((long)ntile * (cls + 1) / 8);
+  int cur = -1;
+  for (int t = lo + (int)(blockIdx.x >> 3); t < hi; t += nper) {
+    int a = 0, b = nsys - 1;   // the last system whose tiles start at or before t
+    while (a < b) {
+      const int m = (a + b + 1) >> 1;
+      if (tile_start[m] <= t) a = m; else b = m - 1;
+    }
+    const SysParams &P = sp[a];
+    if (a != cur) {
+      __syncthreads();   // every wave is done with the previous tables
+      const int ntab = P.ntabA + P.ntabB;
+      for (int u = threadIdx.x; u < ntab; u += PAIR_TB) stab[u] = tab[P.tab0 + u];
+      for (int u = threadIdx.x; u < P.ntype * P.ntype; u += PAIR_TB) slj[u] = ljt[P.lj0 + u];
+      __syncthreads();
+      cur = a;
+    }
+    pair_tile<FIELD, VIRIAL, WATER>(P, t - tile_start[a], stab, slj, qd, qs, gpos, spk, gmeta, nbr,
+                             nnb, nbcap, pf, plj, gacc);
   }
 }
 
@@ -460,20 +537,30 @@ int launch_phi_points(Engine &e, const double4 *sites, int nsite, const double *
   return 0;
 }
 
+int pair_tiles(int ngroup) { return (ngroup + GPB - 1) / GPB; }
+
 int launch_pair(Engine &e, int field_mode, int want_virial) {
-  dim3 grid((e.maxg + GPB - 1) / GPB, e.nsys), tb(PAIR_TB);
-  if (field_mode)
-    hipLaunchKernelGGL((k_pair<true, false>), grid, tb, 0, e.stream, e.d_sp, e.d_gpos, e.d_spk,
-                       e.d_gmeta, e.d_nbr, e.d_nnb, e.nbcap, e.d_tab, e.d_ljt, e.d_pf, e.d_plj,
-                       e.d_gacc);
-  else if (want_virial)
-    hipLaunchKernelGGL((k_pair<false, true>), grid, tb, 0, e.stream, e.d_sp, e.d_gpos, e.d_spk,
-                       e.d_gmeta, e.d_nbr, e.d_nnb, e.nbcap, e.d_tab, e.d_ljt, e.d_pf, e.d_plj,
-                       e.d_gacc);
-  else
-    hipLaunchKernelGGL((k_pair<false, false>), grid, tb, 0, e.stream, e.d_sp, e.d_gpos, e.d_spk,
-                       e.d_gmeta, e.d_nbr, e.d_nnb, e.nbcap, e.d_tab, e.d_ljt, e.d_pf, e.d_plj,
-                       e.d_gacc);
+  if (e.ntile == 0) return 0;
+  dim3 grid(e.pair_blocks), tb(PAIR_TB);
+#define PAIR_ARGS                                                                             \
+  e.d_sp, e.d_tile_start, e.nsys, e.ntile, e.d_gpos, e.d_spk, e.d_gmeta, e.d_nbr, e.d_nnb,  \
+      e.nbcap, e.d_tab, e.d_ljt, e.d_pf, e.d_plj, e.d_gacc
+  // WATER: the kernel carries the path for systems made only of three-site
+  // waters
+#define PAIR_LAUNCH(F, V)                                                                    \
+  if (e.any_water)                                                                           \
+    hipLaunchKernelGGL((k_pair<F, V, true>), grid, tb, 0, e.stream, PAIR_ARGS);              \
+  else                                                                                       \
+    hipLaunchKernelGGL((k_pair<F, V, false>), grid, tb, 0, e.stream, PAIR_ARGS);
+  if (field_mode) {
+    PAIR_LAUNCH(true, false)
+  } else if (want_virial) {
+    PAIR_LAUNCH(false, true)
+  } else {
+    PAIR_LAUNCH(false, false)
+  }
+#undef PAIR_LAUNCH
+#undef PAIR_ARGS
   MM_HIP(hipGetLastError());
   return 0;
 }
