@@ -82,9 +82,11 @@ __global__ void k_nl_scatter(const SysParams *__restrict__ sp, const int *__rest
   order[cell_start[c] + atomicAdd(&cell_fill[c], 1)] = g;
 }
 
-// ascending group id within each cell, so the sorted order is deterministic
+// ascending group id within each cell, so the sorted order is deterministic;
+// then the first-site positions in sorted order (the pair kernel's gpos)
 __global__ void k_nl_sortcell(const SysParams *__restrict__ sp, const int *__restrict__ rebuild,
-                              const int *__restrict__ cell_start, int *__restrict__ order) {
+                              const int *__restrict__ cell_start, int *__restrict__ order,
+                              const double *__restrict__ last, double4 *__restrict__ gpos) {
   const int s = blockIdx.y;
   if (!rebuild[s]) return;
   const SysParams &P = sp[s];
@@ -97,100 +99,124 @@ __global__ void k_nl_sortcell(const SysParams *__restrict__ sp, const int *__res
     while (j >= b && order[j] > v) { order[j + 1] = order[j]; j--; }
     order[j + 1] = v;
   }
+  for (int i = b; i < e; i++) {
+    const int g = order[i];
+    gpos[i] = make_double4(last[3 * g], last[3 * g + 1], last[3 * g + 2], 0.0);
+  }
 }
 
-// one wavefront per sorted group: walk the stencil cells, test candidates,
-// compact the accepted neighbours with a ballot
+// One wavefront per sorted group.  The candidates of up to 64 stencil cells
+// at a time are laid end to end (a wave scan of the cells' counts), each
+// lane takes one candidate, and the accepted ones are compacted with a
+// ballot: the row lists the stencil cells in order and each cell's groups
+// in sorted order.
 __global__ void __launch_bounds__(256) k_nl_pairs(
     const SysParams *__restrict__ sp, const int *__restrict__ rebuild,
     const int *__restrict__ order, const int *__restrict__ gcell,
-    const int *__restrict__ cell_start, const double *__restrict__ last,
-    const int4 *__restrict__ sten, int nbcap, int *__restrict__ nbr, int *__restrict__ nnb,
-    int *__restrict__ err, const double *__restrict__ r, int N, const double *__restrict__ q,
+    const int *__restrict__ cell_start, const int4 *__restrict__ sten, int nbcap,
+    int *__restrict__ nbr, int *__restrict__ nnb, int *__restrict__ err,
+    const double *__restrict__ r, int N, const double *__restrict__ q,
     const int *__restrict__ gfirst, const int *__restrict__ gns, const int *__restrict__ ginfo,
-    double4 *__restrict__ gpos, float4 *__restrict__ spk, int2 *__restrict__ gmeta) {
+    const double4 *__restrict__ gpos, float4 *__restrict__ spk, int2 *__restrict__ gmeta) {
 #pragma clang fp contract(off)
+  __shared__ int c_start[4][64], c_off[4][65], c_cell[4][64];
   const int s = blockIdx.y;
   if (!rebuild[s]) return;
   const SysParams &P = sp[s];
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int *cs = c_start[wv], *co = c_off[wv], *cc = c_cell[wv];
   // grid-stride over sorted groups: a small grid keeps the launch cheap on
   // the many steps that do not rebuild
   const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
   for (int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); wave < P.ngroup;
        wave += nwave) {
-  const int gs = P.group0 + wave;
-  const int g = order[gs];
-  const int n = P.ncell;
-  const int A = gcell[g] - P.cell0;
-  const int ax = A / (n * n), ay = (A / n) % n, az = A % n;
-  const double gx = last[3 * g], gy = last[3 * g + 1], gz = last[3 * g + 2];
-  const double L = P.L, invL = 1 / P.L;
-  const double c2 = P.nl_c2;
-  int cnt = 0;
-  int *row = nbr + (long)gs * nbcap;
-  for (int t0 = 0; t0 < P.nsten; t0 += 64) {
-    const int t = t0 + lane;
-    int bstart = 0, nb = 0, B = -1;
-    if (t < P.nsten) {
-      const int4 o = sten[P.sten0 + t];
-      const int bxx = (ax + o.x) % n, byy = (ay + o.y) % n, bzz = (az + o.z) % n;
-      B = (bxx * n + byy) * n + bzz;
-      bstart = cell_start[P.cell0 + B];
-      nb = cell_start[P.cell0 + B + 1] - bstart;
-    }
-    // the cell that owns the (A,B) row supplies the first argument
-    int ra = A, rb = B;
-    if (B >= 0) canonical_pair(ra, rb);
-    const bool g_first = (ra == A);
-    for (int m = 0; __any(m < nb); m++) {
-      bool take = false;
-      int hs = -1;
-      if (m < nb) {
-        hs = bstart + m;
-        if (hs != gs) {
-          if (B == A) {
-            take = true;
-          } else {
-            const int h = order[hs];
-            const double hx = last[3 * h], hy = last[3 * h + 1], hz = last[3 * h + 2];
-            const double d2 = g_first ? min_image_sq(gx, gy, gz, hx, hy, hz, L, invL)
-                                      : min_image_sq(hx, hy, hz, gx, gy, gz, L, invL);
-            take = d2 < c2;
+    const int gs = P.group0 + wave;
+    const int g = order[gs];
+    const int n = P.ncell;
+    const int A = gcell[g] - P.cell0;
+    const int ax = A / (n * n), ay = (A / n) % n, az = A % n;
+    const double4 pg = gpos[gs];
+    const double L = P.L, invL = 1 / P.L;
+    const double c2 = P.nl_c2;
+    int cnt = 0;
+    int *row = nbr + (long)gs * nbcap;
+    for (int t0 = 0; t0 < P.nsten; t0 += 64) {
+      const int t = t0 + lane;
+      int nb = 0;
+      if (t < P.nsten) {
+        const int4 o = sten[P.sten0 + t];
+        const int bxx = (ax + o.x) % n, byy = (ay + o.y) % n, bzz = (az + o.z) % n;
+        const int B = (bxx * n + byy) * n + bzz;
+        cs[lane] = cell_start[P.cell0 + B];
+        nb = cell_start[P.cell0 + B + 1] - cs[lane];
+        cc[lane] = B;
+      }
+      int incl = nb;   // inclusive scan over the wave
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+      }
+      co[lane + 1] = incl;
+      if (lane == 0) co[0] = 0;
+      __builtin_amdgcn_wave_barrier();
+      const int ncells = min(64, P.nsten - t0);
+      const int total = co[ncells];
+      for (int u0 = 0; u0 < total; u0 += 64) {
+        const int u = u0 + lane;
+        bool take = false;
+        int hs = -1;
+        if (u < total) {
+          int lo = 0, hi = ncells - 1;   // last cell with co[k] <= u
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (co[mid] <= u) lo = mid; else hi = mid - 1;
+          }
+          hs = cs[lo] + (u - co[lo]);
+          const int B = cc[lo];
+          if (hs != gs) {
+            if (B == A) {
+              take = true;
+            } else {
+              // the cell that owns the (A,B) row supplies the first argument
+              int ra = A, rb = B;
+              canonical_pair(ra, rb);
+              const double4 ph = gpos[hs];
+              const double d2 = (ra == A) ? min_image_sq(pg.x, pg.y, pg.z, ph.x, ph.y, ph.z, L, invL)
+                                          : min_image_sq(ph.x, ph.y, ph.z, pg.x, pg.y, pg.z, L, invL);
+              take = d2 < c2;
+            }
           }
         }
+        const unsigned long long bal = __ballot(take);
+        if (take) {
+          const int pos = cnt + __popcll(bal & ((1ull << lane) - 1ull));
+          if (pos < nbcap) row[pos] = hs;
+        }
+        cnt += __popcll(bal);
       }
-      const unsigned long long bal = __ballot(take);
-      if (take) {
-        const int pos = cnt + __popcll(bal & ((1ull << lane) - 1ull));
-        if (pos < nbcap) row[pos] = hs;
+      __builtin_amdgcn_wave_barrier();   // the cell tables are rewritten next
+    }
+    if (lane == 0) {
+      if (cnt > nbcap) {
+        atomicOr(err, ERR_NL_OVERFLOW);
+        cnt = nbcap;
       }
-      cnt += __popcll(bal);
+      nnb[gs] = cnt;
     }
-  }
-  if (lane == 0) {
-    if (cnt > nbcap) {
-      atomicOr(err, ERR_NL_OVERFLOW);
-      cnt = nbcap;
+    // the pair kernel's packed site inputs in the new sorted order (as
+    // k_pack; the first-site positions were written by k_nl_sortcell)
+    const int s0 = gfirst[g], ns = gns[g];
+    if (lane == 0) gmeta[gs] = make_int2(ns, ginfo[g]);
+    if (lane < MAXS) {
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (lane < ns) {
+        const int sk = s0 + lane;
+        v = make_float4((float)(r[sk] - pg.x), (float)(r[N + sk] - pg.y), (float)(r[2 * N + sk] - pg.z),
+                        ((ginfo[g] >> (8 * lane)) & SITE_HAS_Q) ? (float)q[sk] : 0.f);
+      }
+      spk[(long)gs * MAXS + lane] = v;
     }
-    nnb[gs] = cnt;
-  }
-  // the pair kernel's packed inputs in the new sorted order (as k_pack)
-  const int s0 = gfirst[g], ns = gns[g];
-  const double x0 = r[s0], y0 = r[N + s0], z0 = r[2 * N + s0];
-  if (lane == 0) {
-    gpos[gs] = make_double4(x0, y0, z0, 0.0);
-    gmeta[gs] = make_int2(ns, ginfo[g]);
-  }
-  if (lane < MAXS) {
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (lane < ns) {
-      const int sk = s0 + lane;
-      v = make_float4((float)(r[sk] - x0), (float)(r[N + sk] - y0), (float)(r[2 * N + sk] - z0),
-                      ((ginfo[g] >> (8 * lane)) & SITE_HAS_Q) ? (float)q[sk] : 0.f);
-    }
-    spk[(long)gs * MAXS + lane] = v;
-  }
   }
 }
 
@@ -262,12 +288,11 @@ int launch_nl_build(Engine &e, int force) {
   int maxc = 0;
   for (auto &p : e.hsp) maxc = std::max(maxc, p.ncell * p.ncell * p.ncell);
   hipLaunchKernelGGL(k_nl_sortcell, dim3((maxc + tb - 1) / tb, e.nsys), dim3(tb), 0, e.stream,
-                     e.d_sp, e.d_rebuild, e.d_cell_start, e.d_order);
+                     e.d_sp, e.d_rebuild, e.d_cell_start, e.d_order, e.d_last, e.d_gpos);
   hipLaunchKernelGGL(k_nl_pairs, dim3(std::min((e.maxg + 3) / 4, 1024), e.nsys), dim3(256), 0,
-                     e.stream, e.d_sp,
-                     e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start, e.d_last, e.d_sten,
-                     e.nbcap, e.d_nbr, e.d_nnb, e.d_err, e.d_r, e.nsite, e.d_q, e.d_gfirst,
-                     e.d_gns, e.d_ginfo, e.d_gpos, e.d_spk, e.d_gmeta);
+                     e.stream, e.d_sp, e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start,
+                     e.d_sten, e.nbcap, e.d_nbr, e.d_nnb, e.d_err, e.d_r, e.nsite, e.d_q,
+                     e.d_gfirst, e.d_gns, e.d_ginfo, e.d_gpos, e.d_spk, e.d_gmeta);
   MM_HIP(hipGetLastError());
   return 0;
 }
