@@ -150,6 +150,20 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
         cs[lane] = cell_start[P.cell0 + B];
         nb = cell_start[P.cell0 + B + 1] - cs[lane];
         cc[lane] = B;
+        // a cell whose box (cell b holds x in [(b - 1/2) lc, (b + 1/2) lc)
+        // mod L) lies farther than rc + skin from this group holds no pair;
+        // the margin covers the rounding of the cell assignment
+        if (B != A) {
+          const double lc = P.lc, h = 0.5 * P.lc;
+          double dx = pg.x - bxx * lc, dy = pg.y - byy * lc, dz = pg.z - bzz * lc;
+          dx = fabs(dx - L * floor(dx * invL + 0.5)) - h;
+          dy = fabs(dy - L * floor(dy * invL + 0.5)) - h;
+          dz = fabs(dz - L * floor(dz * invL + 0.5)) - h;
+          dx = dx > 0 ? dx : 0;
+          dy = dy > 0 ? dy : 0;
+          dz = dz > 0 ? dz : 0;
+          if (dx * dx + dy * dy + dz * dz > c2 * (1 + 1e-9) + 1e-9) nb = 0;
+        }
       }
       int incl = nb;   // inclusive scan over the wave
 #pragma unroll
