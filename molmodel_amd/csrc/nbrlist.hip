@@ -105,11 +105,14 @@ __global__ void k_nl_sortcell(const SysParams *__restrict__ sp, const int *__res
   }
 }
 
-// One wavefront per sorted group.  The candidates of up to 64 stencil cells
-// at a time are laid end to end (a wave scan of the cells' counts), each
-// lane takes one candidate, and the accepted ones are compacted with a
-// ballot: the row lists the stencil cells in order and each cell's groups
-// in sorted order.
+// One wavefront per sorted group.  The stencil cells that can hold a pair
+// (the group's own cell and those whose box lies within rc + skin of it)
+// are compacted into a per-wavefront list with their counts' prefix sums;
+// their candidates are then laid end to end, each lane takes one, and the
+// accepted ones are compacted with a ballot.  The row lists the stencil
+// cells in order and each cell's groups in sorted order.
+constexpr int NL_SEG = 256;   // listed cells per wavefront before a flush
+
 __global__ void __launch_bounds__(256) k_nl_pairs(
     const SysParams *__restrict__ sp, const int *__restrict__ rebuild,
     const int *__restrict__ order, const int *__restrict__ gcell,
@@ -119,12 +122,13 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
     const int *__restrict__ gfirst, const int *__restrict__ gns, const int *__restrict__ ginfo,
     const double4 *__restrict__ gpos, float4 *__restrict__ spk, int2 *__restrict__ gmeta) {
 #pragma clang fp contract(off)
-  __shared__ int c_start[4][64], c_off[4][65], c_cell[4][64];
+  __shared__ int c_start[4][NL_SEG], c_off[4][NL_SEG + 1], c_cell[4][NL_SEG];
   const int s = blockIdx.y;
   if (!rebuild[s]) return;
   const SysParams &P = sp[s];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int *cs = c_start[wv], *co = c_off[wv], *cc = c_cell[wv];
+  const unsigned long long below = (1ull << lane) - 1ull;
   // grid-stride over sorted groups: a small grid keeps the launch cheap on
   // the many steps that do not rebuild
   const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
@@ -140,48 +144,17 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
     const double c2 = P.nl_c2;
     int cnt = 0;
     int *row = nbr + (long)gs * nbcap;
-    for (int t0 = 0; t0 < P.nsten; t0 += 64) {
-      const int t = t0 + lane;
-      int nb = 0;
-      if (t < P.nsten) {
-        const int4 o = sten[P.sten0 + t];
-        const int bxx = (ax + o.x) % n, byy = (ay + o.y) % n, bzz = (az + o.z) % n;
-        const int B = (bxx * n + byy) * n + bzz;
-        cs[lane] = cell_start[P.cell0 + B];
-        nb = cell_start[P.cell0 + B + 1] - cs[lane];
-        cc[lane] = B;
-        // a cell whose box (cell b holds x in [(b - 1/2) lc, (b + 1/2) lc)
-        // mod L) lies farther than rc + skin from this group holds no pair;
-        // the margin covers the rounding of the cell assignment
-        if (B != A) {
-          const double lc = P.lc, h = 0.5 * P.lc;
-          double dx = pg.x - bxx * lc, dy = pg.y - byy * lc, dz = pg.z - bzz * lc;
-          dx = fabs(dx - L * floor(dx * invL + 0.5)) - h;
-          dy = fabs(dy - L * floor(dy * invL + 0.5)) - h;
-          dz = fabs(dz - L * floor(dz * invL + 0.5)) - h;
-          dx = dx > 0 ? dx : 0;
-          dy = dy > 0 ? dy : 0;
-          dz = dz > 0 ? dz : 0;
-          if (dx * dx + dy * dy + dz * dz > c2 * (1 + 1e-9) + 1e-9) nb = 0;
-        }
-      }
-      int incl = nb;   // inclusive scan over the wave
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-      }
-      co[lane + 1] = incl;
-      if (lane == 0) co[0] = 0;
+    int nseg = 0, tot = 0;   // listed cells and their candidates (wave-uniform)
+    if (lane == 0) co[0] = 0;
+    // test the listed cells' candidates and append the accepted ones
+    auto flush = [&]() {
       __builtin_amdgcn_wave_barrier();
-      const int ncells = min(64, P.nsten - t0);
-      const int total = co[ncells];
-      for (int u0 = 0; u0 < total; u0 += 64) {
+      for (int u0 = 0; u0 < tot; u0 += 64) {
         const int u = u0 + lane;
         bool take = false;
         int hs = -1;
-        if (u < total) {
-          int lo = 0, hi = ncells - 1;   // last cell with co[k] <= u
+        if (u < tot) {
+          int lo = 0, hi = nseg - 1;   // last cell with co[k] <= u
           while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (co[mid] <= u) lo = mid; else hi = mid - 1;
@@ -204,13 +177,58 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
         }
         const unsigned long long bal = __ballot(take);
         if (take) {
-          const int pos = cnt + __popcll(bal & ((1ull << lane) - 1ull));
+          const int pos = cnt + __popcll(bal & below);
           if (pos < nbcap) row[pos] = hs;
         }
         cnt += __popcll(bal);
       }
-      __builtin_amdgcn_wave_barrier();   // the cell tables are rewritten next
+      __builtin_amdgcn_wave_barrier();   // the cell list is rewritten next
+      nseg = 0;
+      tot = 0;
+    };
+    for (int t0 = 0; t0 < P.nsten; t0 += 64) {
+      const int t = t0 + lane;
+      int nb = 0, start = 0, B = -1;
+      if (t < P.nsten) {
+        const int4 o = sten[P.sten0 + t];
+        const int bxx = (ax + o.x) % n, byy = (ay + o.y) % n, bzz = (az + o.z) % n;
+        B = (bxx * n + byy) * n + bzz;
+        start = cell_start[P.cell0 + B];
+        nb = cell_start[P.cell0 + B + 1] - start;
+        // a cell whose box (cell b holds x in [(b - 1/2) lc, (b + 1/2) lc)
+        // mod L) lies farther than rc + skin from this group holds no pair;
+        // the margin covers the rounding of the cell assignment
+        if (B != A) {
+          const double lc = P.lc, h = 0.5 * P.lc;
+          double dx = pg.x - bxx * lc, dy = pg.y - byy * lc, dz = pg.z - bzz * lc;
+          dx = fabs(dx - L * floor(dx * invL + 0.5)) - h;
+          dy = fabs(dy - L * floor(dy * invL + 0.5)) - h;
+          dz = fabs(dz - L * floor(dz * invL + 0.5)) - h;
+          dx = dx > 0 ? dx : 0;
+          dy = dy > 0 ? dy : 0;
+          dz = dz > 0 ? dz : 0;
+          if (dx * dx + dy * dy + dz * dz > c2 * (1 + 1e-9) + 1e-9) nb = 0;
+        }
+      }
+      const unsigned long long keep = __ballot(nb > 0);
+      const int k = __popcll(keep);
+      if (nseg + k > NL_SEG) flush();
+      int incl = nb;   // inclusive scan of the kept counts over the wave
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+      }
+      if (nb > 0) {
+        const int at = nseg + __popcll(keep & below);
+        cs[at] = start;
+        cc[at] = B;
+        co[at + 1] = tot + incl;
+      }
+      tot += __shfl(incl, 63);
+      nseg += k;
     }
+    if (nseg > 0) flush();
     if (lane == 0) {
       if (cnt > nbcap) {
         atomicOr(err, ERR_NL_OVERFLOW);
