@@ -216,8 +216,8 @@ __device__ __forceinline__ void pair_tile(
         // site pairs and the first sites' LJ pair, without per-site mask
         // branches (a tile never spans two systems, so the branch is
         // uniform)
+        const float4 o0 = spk[(long)hs * MAXS];
         {   // first sites: LJ
-          const float4 o0 = spk[(long)hs * MAXS];
           const float rx = Dx - o0.x + oi[0].x, ry = Dy - o0.y + oi[0].y, rz = Dz - o0.z + oi[0].z;
           const float r2 = rx * rx + ry * ry + rz * rz;
           const float2 c = slj[tyi[0] * nty + ((mj.y >> 2) & 0x3f)];
@@ -228,9 +228,12 @@ __device__ __forceinline__ void pair_tile(
           const float du = (12.f * ai12 - 6.f * bi6) * ir2 * smf;
           gx[0] = du * rx; gy[0] = du * ry; gz[0] = du * rz;
         }
+        // site l + 1's offsets are loaded while site l is evaluated
+        float4 ojn = spk[(long)hs * MAXS + 1];
 #pragma unroll 1
         for (int l = 0; l < 3; l++) {
-          const float4 oj = spk[(long)hs * MAXS + l];
+          const float4 oj = l == 0 ? o0 : ojn;
+          if (l == 1) ojn = spk[(long)hs * MAXS + 2];
           const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
           const float qs = oj.w * smf, qsA = qs * T.m2A, qsB = qs * T.m2B;
           float tq = 0.f;
