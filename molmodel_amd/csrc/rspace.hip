@@ -155,6 +155,8 @@ __device__ __forceinline__ void pair_tile(
   float4 *myqd = qd[seg];
   float2 *myqs = qs[seg];
   int qn = 0, t0 = 0;
+  // the row entry of the lane's next candidate, loaded one filter pass ahead
+  int hnext = lane < cnt ? row[lane] : 0;
 
   while (true) {
     // ---- filter until every segment holds a batch or has no candidates
@@ -165,8 +167,9 @@ __device__ __forceinline__ void pair_tile(
       float dsmf = 0.f;
       int hs = 0;
       if (fill && t0 + lane < cnt) {
-        hs = row[t0 + lane];
+        hs = hnext;
         const double4 pj = gpos[hs];
+        if (t0 + LANES + lane < cnt) hnext = row[t0 + LANES + lane];
         double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         dx -= L * floor(dx * invL + 0.5);
         dy -= L * floor(dy * invL + 0.5);
