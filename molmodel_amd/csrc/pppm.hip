@@ -312,6 +312,7 @@ __device__ __forceinline__ void fft_stage(const float2 *xl, float2 *yl, int n, i
 // Stockham autosort FFT of FFT_LINES lines of length n held in x (line l at
 // x + l*FFT_MAXN); every thread of the block must call it.  Radices are
 // taken greedily (4, 2, 3, 5, 7).  Returns the buffer holding the result.
+template <bool POW2>
 __device__ float2 *lds_fft_lines(float2 *x, float2 *y, int n, const float2 *tw, int sign) {
   const int l = threadIdx.x / FFT_T, tid = threadIdx.x % FFT_T;
   int p = 1, rem = n;
@@ -320,6 +321,7 @@ __device__ float2 *lds_fft_lines(float2 *x, float2 *y, int n, const float2 *tw, 
     int R;
     if (rem % 4 == 0) { R = 4; fft_stage<4>(xl, yl, n, p, tw, sign, tid); }
     else if (rem % 2 == 0) { R = 2; fft_stage<2>(xl, yl, n, p, tw, sign, tid); }
+    else if (POW2) { R = rem; }   // not reached: the host launches POW2 only for 2^k lines
     else if (rem % 3 == 0) { R = 3; fft_stage<3>(xl, yl, n, p, tw, sign, tid); }
     else if (rem % 5 == 0) { R = 5; fft_stage<5>(xl, yl, n, p, tw, sign, tid); }
     else { R = 7; fft_stage<7>(xl, yl, n, p, tw, sign, tid); }
@@ -337,6 +339,7 @@ __device__ __forceinline__ void load_tw(float2 *stw, const float2 *tw, int n) {
 
 // pass 0: z lines, real -> half spectrum.  pass 1/3: y lines fwd/inv.
 // pass 4: z lines, half spectrum -> real.
+template <bool POW2>
 __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restrict__ sp,
                                                      const float2 *__restrict__ twid_all,
                                                      float *__restrict__ mesh,
@@ -382,7 +385,7 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
     }
   }
   __syncthreads();
-  float2 *res = lds_fft_lines(bx_, by_, n, stw, sign);
+  float2 *res = lds_fft_lines<POW2>(bx_, by_, n, stw, sign);
   float2 *rl = res + l * FFT_MAXN;
   if (!on) return;
   if (pass == 0) {
@@ -401,7 +404,7 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
 // x lines: forward FFT, multiply by the influence function, inverse FFT.
 // Also the mesh energy 1/2 sum G |rho_k|^2 and its virial
 // W_ab = sum e_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 eta^2))).
-template <bool VIR>
+template <bool VIR, bool POW2>
 __global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__restrict__ sp,
                                                        const float2 *__restrict__ twid_all,
                                                        const float *__restrict__ green,
@@ -435,7 +438,7 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__res
   if (on)
     for (int t = tid; t < nx; t += FFT_T) xl[t] = src[t * xstride];
   __syncthreads();
-  float2 *res = lds_fft_lines(bx_, by_, nx, stw, -1);
+  float2 *res = lds_fft_lines<POW2>(bx_, by_, nx, stw, -1);
   float2 *oth = (res == bx_) ? by_ : bx_;
   double e = 0, w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (on) {
@@ -462,7 +465,7 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__res
     }
   }
   __syncthreads();
-  res = lds_fft_lines(res, oth, nx, stw, +1);
+  res = lds_fft_lines<POW2>(res, oth, nx, stw, +1);
   if (on) {
     float2 *rl = res + l * FFT_MAXN;
     for (int t = tid; t < nx; t += FFT_T) src[t * xstride] = rl[t];
@@ -534,6 +537,27 @@ __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict_
   kres[4 * si + 3] = (double)e0 * T[6] + (double)e1 * T[7] + (double)e2 * T[8];
 }
 
+// forward z, y passes; x pass with the influence function; inverse y, z
+template <bool POW2>
+static void launch_fft(Engine &e, int want_virial, hipStream_t st, int maxx, int maxy, int maxz) {
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+                     e.d_twid, e.d_mesh, e.d_cmesh, 0);
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+                     e.d_twid, e.d_mesh, e.d_cmesh, 1);
+  if (want_virial)
+    hipLaunchKernelGGL((k_fft_xgreen<true, POW2>), dim3(maxx, e.nsys), dim3(FFT_TB), 0, st,
+                       e.d_sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
+                       e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
+  else
+    hipLaunchKernelGGL((k_fft_xgreen<false, POW2>), dim3(maxx, e.nsys), dim3(FFT_TB), 0, st,
+                       e.d_sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
+                       e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+                     e.d_twid, e.d_mesh, e.d_cmesh, 3);
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+                     e.d_twid, e.d_mesh, e.d_cmesh, 4);
+}
+
 int launch_p3m(Engine &e, int want_virial, hipStream_t st) {
   int maxn = 0, maxbins = 0, maxz = 0, maxy = 0, maxx = 0;
   for (auto &p : e.hsp)
@@ -563,22 +587,14 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st) {
                        e.d_sp, e.d_bin_start, e.d_bgrid, e.d_bfrac, e.d_mesh);
   SPREAD(1) SPREAD(2) SPREAD(3) SPREAD(4) SPREAD(5) SPREAD(6) SPREAD(7)
 #undef SPREAD
-  hipLaunchKernelGGL(k_fft_pass, dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
-                     e.d_mesh, e.d_cmesh, 0);
-  hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
-                     e.d_mesh, e.d_cmesh, 1);
-  if (want_virial)
-    hipLaunchKernelGGL(k_fft_xgreen<true>, dim3(maxx, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
-                       e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
-                       e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
-  else
-    hipLaunchKernelGGL(k_fft_xgreen<false>, dim3(maxx, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
-                       e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
-                       e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
-  hipLaunchKernelGGL(k_fft_pass, dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
-                     e.d_mesh, e.d_cmesh, 3);
-  hipLaunchKernelGGL(k_fft_pass, dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp, e.d_twid,
-                     e.d_mesh, e.d_cmesh, 4);
+  // meshes whose sides are all powers of two take FFT kernels built with
+  // the radix-4/2 stages only (fewer registers)
+  bool pow2 = true;
+  for (auto &p : e.hsp)
+    if (p.kmode == 1)
+      for (int n : {p.nx, p.ny, p.nz}) pow2 &= (n & (n - 1)) == 0;
+  if (pow2) launch_fft<true>(e, want_virial, st, maxx, maxy, maxz);
+  else launch_fft<false>(e, want_virial, st, maxx, maxy, maxz);
 #define GATHER(PO)                                                                        \
   if (used[PO])                                                                           \
     hipLaunchKernelGGL(k_p3m_gather<PO>, gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid,     \
