@@ -236,14 +236,16 @@ static int compute_checked(mm_engine *m, int field_mode, int want_virial) {
     if (!field_mode) TRY(launch_finish_ex(e, FIN_FORCES, want_virial, e.d_phi));
     int f = 0;
     TRY(read_err(e, &f));
+    if (f & (ERR_NAN | ERR_CONSTRAINT)) {   // reported once, before any retry
+      TRY(clear_err(e));
+      set_last_error(f & ERR_NAN ? "non-finite position or energy during force evaluation"
+                                 : "device error flag raised during force evaluation");
+      return MM_ERR_DEVICE;
+    }
     if (f & ERR_NL_OVERFLOW) {
       TRY(clear_err(e));
       TRY(grow_nbr(e));
       continue;
-    }
-    if (f & (ERR_NAN | ERR_CONSTRAINT)) {
-      set_last_error("device error flag raised during force evaluation");
-      return MM_ERR_DEVICE;
     }
     return 0;
   }
@@ -552,6 +554,7 @@ static int md_check_errors(Engine &e) {
   int f = 0;
   TRY(read_err(e, &f));
   if (f) {
+    TRY(clear_err(e));   // reported once
     set_last_error(f & ERR_NL_OVERFLOW ? "neighbour list overflow during MD"
                    : f & ERR_CONSTRAINT ? "constraint iteration did not converge"
                                         : "non-finite energy during MD");

@@ -34,7 +34,7 @@ __global__ void k_nl_assign(const SysParams *__restrict__ sp, const int *__restr
                             const double *__restrict__ r, int nsite_total,
                             const int *__restrict__ rebuild, int *__restrict__ gcell,
                             double *__restrict__ last, int *__restrict__ cell_count,
-                            unsigned long long *__restrict__ builds) {
+                            unsigned long long *__restrict__ builds, int *__restrict__ err) {
 #pragma clang fp contract(off)
   const int s = blockIdx.y;
   if (!rebuild[s]) return;
@@ -46,10 +46,15 @@ __global__ void k_nl_assign(const SysParams *__restrict__ sp, const int *__restr
   const int site = gfirst[g];
   const double x = r[site], y = r[nsite_total + site], z = r[2 * nsite_total + site];
   const int n = P.ncell;
-  const int ix = (int)floor(x / P.lc + 0.5);
-  const int iy = (int)floor(y / P.lc + 0.5);
-  const int iz = (int)floor(z / P.lc + 0.5);
-  const int c = P.cell0 + (mod_nonneg(ix, n) * n + mod_nonneg(iy, n)) * n + mod_nonneg(iz, n);
+  int c = P.cell0;
+  if (fabs(x) <= 1e300 && fabs(y) <= 1e300 && fabs(z) <= 1e300) {
+    const int ix = (int)floor(x / P.lc + 0.5);
+    const int iy = (int)floor(y / P.lc + 0.5);
+    const int iz = (int)floor(z / P.lc + 0.5);
+    c += (mod_nonneg(ix, n) * n + mod_nonneg(iy, n)) * n + mod_nonneg(iz, n);
+  } else {
+    atomicOr(err, ERR_NAN);   // reported; the group is parked in cell 0
+  }
   gcell[g] = c;
   last[3 * g] = x;
   last[3 * g + 1] = y;
@@ -86,9 +91,14 @@ __global__ void k_nl_scatter(const SysParams *__restrict__ sp, const int *__rest
 // then the first-site positions in sorted order (the pair kernel's gpos)
 __global__ void k_nl_sortcell(const SysParams *__restrict__ sp, const int *__restrict__ rebuild,
                               const int *__restrict__ cell_start, int *__restrict__ order,
-                              const double *__restrict__ last, double4 *__restrict__ gpos) {
+                              const double *__restrict__ last, double4 *__restrict__ gpos,
+                              const int *__restrict__ err) {
   const int s = blockIdx.y;
   if (!rebuild[s]) return;
+  // after a non-finite position (its group parked in cell 0) skip the sort,
+  // whose serial cost grows with the square of a cell's population; the
+  // rows built from the unsorted order are discarded with the error
+  if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_NAN) return;
   const SysParams &P = sp[s];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.ncell * P.ncell * P.ncell) return;
@@ -269,7 +279,8 @@ __global__ void k_skin(const SysParams *__restrict__ sp, const int *__restrict__
     const int g = P.group0 + i, st = gfirst[g];
     const double dx = r[st] - last[3 * g], dy = r[N + st] - last[3 * g + 1],
                  dz = r[2 * N + st] - last[3 * g + 2];
-    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(d <= 1e300)) d = 1e300;   // a non-finite position (or a list built from one) rebuilds
     if (d > t1) { t2 = t1; t1 = d; } else if (d > t2) t2 = d;
   }
   __shared__ double a1[16], a2[16];
@@ -312,7 +323,7 @@ int launch_nl_build(Engine &e, int force) {
   dim3 gg((e.maxg + tb - 1) / tb, e.nsys);
   const double *r = e.d_r;
   hipLaunchKernelGGL(k_nl_assign, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_gfirst, r, e.nsite,
-                     e.d_rebuild, e.d_gcell, e.d_last, e.d_cell_count, e.d_nlb);
+                     e.d_rebuild, e.d_gcell, e.d_last, e.d_cell_count, e.d_nlb, e.d_err);
   hipLaunchKernelGGL(k_nl_scan, dim3(1, e.nsys), dim3(1024), 0, e.stream, e.d_sp, e.d_rebuild,
                      e.d_cell_count, e.d_cell_start, e.d_cell_fill);
   hipLaunchKernelGGL(k_nl_scatter, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_rebuild, e.d_gcell,
@@ -320,7 +331,8 @@ int launch_nl_build(Engine &e, int force) {
   int maxc = 0;
   for (auto &p : e.hsp) maxc = std::max(maxc, p.ncell * p.ncell * p.ncell);
   hipLaunchKernelGGL(k_nl_sortcell, dim3((maxc + tb - 1) / tb, e.nsys), dim3(tb), 0, e.stream,
-                     e.d_sp, e.d_rebuild, e.d_cell_start, e.d_order, e.d_last, e.d_gpos);
+                     e.d_sp, e.d_rebuild, e.d_cell_start, e.d_order, e.d_last, e.d_gpos,
+                     e.d_err);
   hipLaunchKernelGGL(k_nl_pairs, dim3(std::min((e.maxg + 3) / 4, 1024), e.nsys), dim3(256), 0,
                      e.stream, e.d_sp, e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start,
                      e.d_sten, e.nbcap, e.d_nbr, e.d_nnb, e.d_err, e.d_r, e.nsite, e.d_q,
