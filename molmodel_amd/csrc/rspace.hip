@@ -82,6 +82,7 @@ constexpr int QCAP = 32;      // queue entries per i-group (one batch of 16 + a 
 // index is min(x_A, x_B) and the segment is the one giving the minimum.
 struct TabView {
   float amin, bmax, ainv, binv, offA, offB, m2A, m2B, eta;
+  unsigned ntab;   // intervals in both segments
 };
 
 __device__ __forceinline__ void coul_pair(const TabView &T, const float4 *stab, float r2, float qs,
@@ -104,6 +105,27 @@ __device__ __forceinline__ void coul_pair(const TabView &T, const float4 *stab, 
     b0 = erfcf(er) / rm;
     e = qs * (b0 + T.eta * 1.1283791670955126f * expf(-er * er)) / r2;
   }
+}
+
+// The table branch of coul_pair without the range test: the interval index
+// is clamped into the table and the largest unclamped index (as unsigned, so
+// that an index below the table is large too) is kept in worst; a caller
+// whose worst reaches T.ntab evaluates the group pair again with coul_pair.
+// Inside the range the arithmetic is coul_pair's.
+__device__ __forceinline__ void coul_table(const TabView &T, const float4 *stab, float r2,
+                                           float qsA, float qsB, float &b0, float &e,
+                                           unsigned &worst) {
+  const float xa = fmaf(r2, T.ainv, T.offA), xb = fmaf(r2, T.binv, T.offB);
+  const bool inA = xa < xb;
+  const float xx = inA ? xa : xb;
+  const float fl = floorf(xx);
+  const unsigned u = (unsigned)(int)fl;
+  worst = u > worst ? u : worst;
+  const float fr = xx - fl;
+  const float4 c = stab[u < T.ntab - 1u ? u : T.ntab - 1u];
+  const float p2 = fmaf(c.w, fr, c.z), p1 = fmaf(p2, fr, c.y);
+  b0 = fmaf(p1, fr, c.x);
+  e = fmaf(fr, fmaf(c.w, fr, p2), p1) * (inA ? qsA : qsB);
 }
 
 // One tile: the GPB i-groups blk * GPB ... of system P.  The block's LDS
@@ -149,6 +171,7 @@ __device__ __forceinline__ void pair_tile(
   T.offB = (float)P.ntabA - P.tabMid * P.tabBinv;
   T.m2A = -2.f * P.tabAinv; T.m2B = -2.f * P.tabBinv;
   T.eta = (float)P.eta;
+  T.ntab = (unsigned)(P.ntabA + P.ntabB);
   const int nty = P.ntype;
   const int cnt = active ? nnb[gs] : 0;
   const int *row = nbr + (long)gs * nbcap;
@@ -214,11 +237,15 @@ __device__ __forceinline__ void pair_tile(
 #pragma unroll
       for (int k = 0; k < NSI; k++) { fx[k] = fy[k] = fz[k] = ph[k] = 0.f; gx[k] = gy[k] = gz[k] = 0.f; }
       float tmps = 0.f, uu = 0.f;
+      bool general = true;
       if (WATER && P.water) {
         // every group of this system is a three-site water: nine charged
         // site pairs and the first sites' LJ pair, without per-site mask
         // branches (a tile never spans two systems, so the branch is
-        // uniform)
+        // uniform) and without the table's range test; a lane with a site
+        // pair outside the table (closer than the table's start) takes the
+        // general path below for this group pair
+        unsigned worst = 0;
         const float4 o0 = spk[(long)hs * MAXS];
         {   // first sites: LJ
           const float rx = Dx - o0.x + oi[0].x, ry = Dy - o0.y + oi[0].y, rz = Dz - o0.z + oi[0].z;
@@ -245,14 +272,24 @@ __device__ __forceinline__ void pair_tile(
             const float rx = px + oi[k].x, ry = py + oi[k].y, rz = pz + oi[k].z;
             const float r2 = rx * rx + ry * ry + rz * rz;
             float b0, e;
-            coul_pair(T, stab, r2, qs, qsA, qsB, b0, e);
+            coul_table(T, stab, r2, qsA, qsB, b0, e, worst);
             tq += oi[k].w * b0;
             ph[k] += qs * b0;
             fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
           }
           tmps += oj.w * tq;
         }
-      } else {
+        general = worst >= T.ntab;
+        if (general) {
+#pragma unroll
+          for (int k = 0; k < NSI; k++) {
+            fx[k] = fy[k] = fz[k] = ph[k] = 0.f;
+            gx[k] = gy[k] = gz[k] = 0.f;
+          }
+          tmps = uu = 0.f;
+        }
+      }
+      if (general) {
 #pragma unroll 1
       for (int l = 0; l < nsj; l++) {
         const float4 oj = spk[(long)hs * MAXS + l];

@@ -87,3 +87,50 @@ def test_forces_energy_virial(gpu_lib, name):
     assert abs(en[1] - oen[1]) <= 1e-5 * abs(oen[3])
     assert abs(en[2] - oen[2]) <= 1e-12 * abs(oen[2])
     assert rel(w, ow.reshape(3, 3)) <= 1e-5
+
+
+def close_contact(s, gap=0.85):
+    """Positions of s with one water translated so that one of its H sites
+    lies gap Å from the O of group 0 (r^2 below the erfc table's start,
+    0.81 Å^2, so that site pair is evaluated with erfc/exp) and no other
+    pair of sites of different groups is closer than 1 Å."""
+    L = s.boxl
+    r = s.r.copy()
+    o0 = r[0]
+    g = s.group
+    d0 = r[g[:, 0]] - o0
+    d0 -= L * np.floor(d0 / L + 0.5)
+    for k in np.argsort((d0 ** 2).sum(1))[1:40]:
+        a, b = g[k]
+        bond = r[a + 1] - r[a]
+        shift = (o0 - gap * bond / np.linalg.norm(bond)) - r[a + 1]
+        t = r.copy()
+        t[a:b] += shift
+        others = np.ones(s.nsite, bool)
+        others[a:b] = False
+        dm = t[others][None, :, :] - t[a:b][:, None, :]
+        dm -= L * np.floor(dm / L + 0.5)
+        dist = np.sqrt((dm ** 2).sum(-1))
+        if np.sort(dist.ravel())[1] >= 1.0:
+            assert abs(dist.min() - gap) < 1e-9
+            return t
+    raise AssertionError("no water could be placed")
+
+
+def test_site_pair_below_table_matches_oracle(gpu_lib):
+    s, p = W.config("water648_ewald")
+    r = close_contact(s)
+    e = Engine(s, p)
+    e.set_positions(r)
+    print("[close] H-O pair at 0.85 A", flush=True)
+    phi, ev, ljf, lje = e.rspace_field()
+    ophi, oev, oljf, olje = O.rspace_parts(s, p, r)
+    print("close-contact rspace rel errors", rel(phi, ophi), rel(ev, oev), rel(ljf, oljf))
+    assert rel(phi, ophi) < 2e-6
+    assert rel(ev, oev) < 5e-6
+    assert rel(ljf, oljf) < 5e-6
+    e.compute()
+    f = e.forces()
+    of, oen = O.evaluate(s, p, r)
+    print(f"close-contact force rel {rel(f, of):.3e}")
+    assert rel(f, of) <= 1e-5
