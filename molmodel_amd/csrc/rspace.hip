@@ -258,12 +258,10 @@ __device__ __forceinline__ void pair_tile(
           const float du = (12.f * ai12 - 6.f * bi6) * ir2 * smf;
           gx[0] = du * rx; gy[0] = du * ry; gz[0] = du * rz;
         }
-        // site l + 1's offsets are loaded while site l is evaluated
-        float4 ojn = spk[(long)hs * MAXS + 1];
-#pragma unroll 1
+        const float4 o1 = spk[(long)hs * MAXS + 1], o2 = spk[(long)hs * MAXS + 2];
+#pragma unroll
         for (int l = 0; l < 3; l++) {
-          const float4 oj = l == 0 ? o0 : ojn;
-          if (l == 1) ojn = spk[(long)hs * MAXS + 2];
+          const float4 oj = l == 0 ? o0 : l == 1 ? o1 : o2;
           const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
           const float qs = oj.w * smf, qsA = qs * T.m2A, qsB = qs * T.m2B;
           float tq = 0.f;
