@@ -118,10 +118,11 @@ __device__ __forceinline__ void coul_table(const TabView &T, const float4 *stab,
   const float xa = fmaf(r2, T.ainv, T.offA), xb = fmaf(r2, T.binv, T.offB);
   const bool inA = xa < xb;
   const float xx = inA ? xa : xb;
-  const float fl = floorf(xx);
-  const unsigned u = (unsigned)(int)fl;
+  int iu;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(iu) : "v"(xx));   // floor, as an integer
+  const unsigned u = (unsigned)iu;
   worst = u > worst ? u : worst;
-  const float fr = xx - fl;
+  const float fr = __builtin_amdgcn_fractf(xx);
   const float4 c = stab[u < T.ntab - 1u ? u : T.ntab - 1u];
   const float p2 = fmaf(c.w, fr, c.z), p1 = fmaf(p2, fr, c.y);
   b0 = fmaf(p1, fr, c.x);
