@@ -234,10 +234,9 @@ __device__ __forceinline__ void pair_tile(
       const int2 mj = gmeta[hs];
       const int nsj = mj.x;
       const float Dx = ent.x, Dy = ent.y, Dz = ent.z;
+      // per group pair FP32 sums (set by the path that runs)
       float fx[NSI], fy[NSI], fz[NSI], ph[NSI], gx[NSI], gy[NSI], gz[NSI];
-#pragma unroll
-      for (int k = 0; k < NSI; k++) { fx[k] = fy[k] = fz[k] = ph[k] = 0.f; gx[k] = gy[k] = gz[k] = 0.f; }
-      float tmps = 0.f, uu = 0.f;
+      float tmps, uu;
       bool general = true;
       if (WATER && P.water) {
         // every group of this system is a three-site water: nine charged
@@ -258,6 +257,8 @@ __device__ __forceinline__ void pair_tile(
           uu = ai12 - bi6;
           const float du = (12.f * ai12 - 6.f * bi6) * ir2 * smf;
           gx[0] = du * rx; gy[0] = du * ry; gz[0] = du * rz;
+#pragma unroll
+          for (int k = 1; k < NSI; k++) gx[k] = gy[k] = gz[k] = 0.f;
         }
         const float4 o1 = spk[(long)hs * MAXS + 1], o2 = spk[(long)hs * MAXS + 2];
 #pragma unroll
@@ -265,30 +266,31 @@ __device__ __forceinline__ void pair_tile(
           const float4 oj = l == 0 ? o0 : l == 1 ? o1 : o2;
           const float px = Dx - oj.x, py = Dy - oj.y, pz = Dz - oj.z;
           const float qs = oj.w * smf, qsA = qs * T.m2A, qsB = qs * T.m2B;
-          float tq = 0.f;
 #pragma unroll
           for (int k = 0; k < 3; k++) {
             const float rx = px + oi[k].x, ry = py + oi[k].y, rz = pz + oi[k].z;
             const float r2 = rx * rx + ry * ry + rz * rz;
             float b0, e;
             coul_table(T, stab, r2, qsA, qsB, b0, e, worst);
-            tq += oi[k].w * b0;
-            ph[k] += qs * b0;
-            fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
+            if (l == 0) {   // the sums start with site 0's terms
+              ph[k] = oj.w * b0;   // unsmoothed here, smoothed below
+              fx[k] = e * rx; fy[k] = e * ry; fz[k] = e * rz;
+            } else {
+              ph[k] += oj.w * b0;
+              fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
+            }
           }
-          tmps += oj.w * tq;
         }
-        general = worst >= T.ntab;
-        if (general) {
+        // unsmoothed pair energy over the charges, then the smoothed potentials
+        tmps = oi[0].w * ph[0] + oi[1].w * ph[1] + oi[2].w * ph[2];
 #pragma unroll
-          for (int k = 0; k < NSI; k++) {
-            fx[k] = fy[k] = fz[k] = ph[k] = 0.f;
-            gx[k] = gy[k] = gz[k] = 0.f;
-          }
-          tmps = uu = 0.f;
-        }
+        for (int k = 0; k < 3; k++) ph[k] *= smf;
+        general = worst >= T.ntab;
       }
       if (general) {
+#pragma unroll
+        for (int k = 0; k < NSI; k++) { fx[k] = fy[k] = fz[k] = ph[k] = 0.f; gx[k] = gy[k] = gz[k] = 0.f; }
+        tmps = uu = 0.f;
 #pragma unroll 1
       for (int l = 0; l < nsj; l++) {
         const float4 oj = spk[(long)hs * MAXS + l];
