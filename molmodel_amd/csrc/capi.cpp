@@ -189,7 +189,9 @@ static int grow_nbr(Engine &e) {
   return 0;
 }
 
-static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
+// nl_from_partials: the last finish left per-block partial sums (FIN_LAZY)
+static int enqueue_forces(mm_engine *m, int field_mode, int want_virial,
+                          int nl_from_partials = 0) {
   Engine &e = m->e;
   if (e.no_rspace) {
     TRY(launch_kspace(e, want_virial));
@@ -216,7 +218,7 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial) {
     e.first_eval = 0;
   } else {
     if (m->pos_dirty) TRY(launch_skin(e));
-    TRY(launch_nl_build(e, 0));
+    TRY(launch_nl_build(e, 0, nl_from_partials));
   }
   if (e.need_pack || m->pos_dirty) TRY(launch_pack(e));
   e.need_pack = 0;
@@ -513,8 +515,10 @@ int mm_md_setup(mm_engine *m, double dt, double bond, double angle_deg, double t
   return MM_OK;
 }
 
+// one MD step after a finish; inner steps leave partial sums (FIN_LAZY),
+// which the next step's neighbour-list check reduces itself
 static int enqueue_step(mm_engine *m, int mode) {
-  TRY(enqueue_forces(m, 0, 0));
+  TRY(enqueue_forces(m, 0, 0, 1));
   TRY(launch_finish_ex(m->e, mode, 0, m->e.d_phi));
   return 0;
 }
@@ -532,7 +536,7 @@ static int md_run_async(mm_engine *m, int nsteps, int use_graph) {
     if (!e.gexec) {
       // the step graph: neighbour-list check/rebuild, forces, finish
       MM_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_step(m, FIN_KICK | FIN_DRIFT);
+      int rc = enqueue_step(m, FIN_KICK | FIN_DRIFT | FIN_LAZY);
       hipGraph_t g = nullptr;
       hipError_t ce = hipStreamEndCapture(e.stream, &g);
       if (rc) return rc;
@@ -542,7 +546,7 @@ static int md_run_async(mm_engine *m, int nsteps, int use_graph) {
     }
     for (int s = 0; s < full; s++) MM_HIP(hipGraphLaunch(e.gexec, e.stream));
   } else {
-    for (int s = 0; s < full; s++) TRY(enqueue_step(m, FIN_KICK | FIN_DRIFT));
+    for (int s = 0; s < full; s++) TRY(enqueue_step(m, FIN_KICK | FIN_DRIFT | FIN_LAZY));
   }
   // last step ends synchronised: r, v, F of the same time
   TRY(enqueue_step(m, FIN_KICK));

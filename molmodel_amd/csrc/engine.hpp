@@ -161,7 +161,9 @@ void lj_tail_factors(const SystemInput &s, double *ca, double *cb);
 int engine_scale_system(Engine &e, int s, double f);
 
 // launchers (each returns 0 or a negative status)
-int launch_nl_build(Engine &e, int force);
+// from_partials: the rebuild test of each system also fires when the
+// displacement partials of the last (FIN_LAZY) finish exceed the skin
+int launch_nl_build(Engine &e, int force, int from_partials = 0);
 int launch_pack(Engine &e);
 int launch_pair(Engine &e, int field_mode, int want_virial);
 int pair_tiles(int ngroup);
@@ -190,6 +192,16 @@ enum : int {   // launch_finish modes
   FIN_FORCES = 0,        // forces + energies only
   FIN_KICK = 1,          // + v += F dt/2m, RATTLE velocities, kinetic energy
   FIN_DRIFT = 2,         // + v += F dt/2m, r += v dt, SHAKE positions
+  FIN_LAZY = 4,          // per-block partial sums only: the per-system totals
+                         // (energies, displacements) are not formed, the
+                         // rebuild flags are cleared, and the next
+                         // neighbour-list check reduces the displacement
+                         // partials itself (launch_nl_build from_partials)
 };
+// finish kernel blocks and the layout of their partial sums without the
+// virial: elec, lj, kin, largest and second largest displacement
+constexpr int FIN_TB = 256;
+constexpr int FIN_NPART = 5;
+constexpr int FIN_TOP = 3;
 
 }  // namespace mmamd

@@ -24,14 +24,15 @@
 
 namespace mmamd {
 
-constexpr int FIN_TB = 256;
 // reduced per system: elec, lj, kin, [vir[9]], top1, top2
 template <bool VIR> struct FinSlots {
   static constexpr int N = VIR ? 14 : 5;
   static constexpr int TOP = N - 2;
 };
+static_assert(FinSlots<false>::N == FIN_NPART && FinSlots<false>::TOP == FIN_TOP,
+              "the neighbour-list check reads this layout");
 
-template <int MODE, bool VIR>
+template <int MODE, bool VIR, bool LAZY>
 __global__ void __launch_bounds__(FIN_TB) k_finish(
     const SysParams *__restrict__ sp, const int *__restrict__ order, const int *__restrict__ gfirst,
     const int *__restrict__ gns, const int *__restrict__ gexcl, const double *__restrict__ q,
@@ -187,6 +188,15 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
                  dz = x[0][2] - last[3 * g + 2];
     vals[TOP] = sqrt(dx * dx + dy * dy + dz * dz);
   }
+  if constexpr (LAZY) {
+    // partials only; a non-finite block sum is a non-finite total
+    store_partials<NPART, TOP, FIN_TB>(vals, part + (long)s * maxblk * NPART, blockIdx.x, red);
+    if (threadIdx.x == 0) {
+      if (!(vals[0] == vals[0]) || !(vals[1] == vals[1])) atomicOr(err, ERR_NAN);
+      if (blockIdx.x == 0) rebuild[s] = 0;   // the next check decides
+    }
+    return;
+  }
   if (finish_partials<NPART, TOP, FIN_TB>(vals, part + (long)s * maxblk * NPART, nblk, blockIdx.x,
                                          &done[s], red, &islast)) {
     double *a = acc + s * NACC;
@@ -290,14 +300,23 @@ int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out) {
     e.d_err, e.d_gpos, e.d_spk, e.d_ginfo
 #define FIN_CASE(M)                                                                            \
   if (want_virial)                                                                             \
-    hipLaunchKernelGGL((k_finish<M, true>), grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS);         \
+    hipLaunchKernelGGL((k_finish<M, true, false>), grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS);  \
   else                                                                                         \
-    hipLaunchKernelGGL((k_finish<M, false>), grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS);
-  switch (mode) {
-    case 0: FIN_CASE(0) break;
-    case 1: FIN_CASE(1) break;
-    case 2: FIN_CASE(2) break;
-    default: FIN_CASE(3) break;
+    hipLaunchKernelGGL((k_finish<M, false, false>), grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS);
+  if (mode & FIN_LAZY) {
+    if (want_virial || (mode & ~FIN_LAZY) != (FIN_KICK | FIN_DRIFT)) {
+      set_last_error("partial-sum finish is built for the MD step without virial only");
+      return -1;
+    }
+    hipLaunchKernelGGL((k_finish<FIN_KICK | FIN_DRIFT, false, true>), grid, dim3(FIN_TB), 0,
+                       e.stream, FIN_ARGS);
+  } else {
+    switch (mode) {
+      case 0: FIN_CASE(0) break;
+      case 1: FIN_CASE(1) break;
+      case 2: FIN_CASE(2) break;
+      default: FIN_CASE(3) break;
+    }
   }
 #undef FIN_CASE
 #undef FIN_ARGS

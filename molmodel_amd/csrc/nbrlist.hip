@@ -15,6 +15,7 @@
 // neighbour indices in a fixed-capacity row, for the pair kernel.
 #include "engine.hpp"
 #include "scan.hpp"
+#include "reduce.hpp"
 
 #pragma clang fp contract(off)
 
@@ -30,15 +31,46 @@ __device__ __forceinline__ double min_image_sq(double ax, double ay, double az, 
   return x * x + y * y + z * z;
 }
 
-__global__ void k_nl_assign(const SysParams *__restrict__ sp, const int *__restrict__ gfirst,
-                            const double *__restrict__ r, int nsite_total,
-                            const int *__restrict__ rebuild, int *__restrict__ gcell,
-                            double *__restrict__ last, int *__restrict__ cell_count,
-                            unsigned long long *__restrict__ builds, int *__restrict__ err) {
+// from_part: the skin test (nbrlist.cpp:64-76) on the largest and second
+// largest first-site displacement since the last build, merged from the
+// per-block partials of the finish kernel that moved the positions (every
+// block forms the same values); a rebuild already requested stays
+// requested, and block 0 records the outcome for the rest of the chain.
+__global__ void __launch_bounds__(256) k_nl_assign(
+    const SysParams *__restrict__ sp, const int *__restrict__ gfirst,
+    const double *__restrict__ r, int nsite_total, int *__restrict__ rebuild,
+    int *__restrict__ gcell, double *__restrict__ last, int *__restrict__ cell_count,
+    unsigned long long *__restrict__ builds, int *__restrict__ err,
+    const double *__restrict__ part, int maxblk, int from_part) {
 #pragma clang fp contract(off)
   const int s = blockIdx.y;
-  if (!rebuild[s]) return;
   const SysParams &P = sp[s];
+  int want = rebuild[s];
+  if (from_part) {
+    __shared__ double w1[4], w2[4];
+    __shared__ int fire;
+    const int nb = (P.ngroup + FIN_TB - 1) / FIN_TB;
+    double t1 = 0, t2 = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      const double *q = part + ((long)s * maxblk + b) * FIN_NPART + FIN_TOP;
+      top2_merge(t1, t2, load_coherent(q), load_coherent(q + 1));
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double b1 = __shfl_xor(t1, o), b2 = __shfl_xor(t2, o);
+      top2_merge(t1, t2, b1, b2);
+    }
+    if ((threadIdx.x & 63) == 0) { w1[threadIdx.x >> 6] = t1; w2[threadIdx.x >> 6] = t2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double u1 = w1[0], u2 = w2[0];
+      for (int k = 1; k < (int)(blockDim.x >> 6); k++) top2_merge(u1, u2, w1[k], w2[k]);
+      fire = u1 + u2 > P.skin;   // nbrlist.cpp:75-76
+    }
+    __syncthreads();
+    want |= fire;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && fire) rebuild[s] = 1;
+  }
+  if (!want) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) builds[s] += 1;
   if (i >= P.ngroup) return;
@@ -314,7 +346,7 @@ int launch_set_rebuild(Engine &e, int value) {
   return 0;
 }
 
-int launch_nl_build(Engine &e, int force) {
+int launch_nl_build(Engine &e, int force, int from_partials) {
   if (force) {
     int rc = launch_set_rebuild(e);
     if (rc) return rc;
@@ -323,7 +355,8 @@ int launch_nl_build(Engine &e, int force) {
   dim3 gg((e.maxg + tb - 1) / tb, e.nsys);
   const double *r = e.d_r;
   hipLaunchKernelGGL(k_nl_assign, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_gfirst, r, e.nsite,
-                     e.d_rebuild, e.d_gcell, e.d_last, e.d_cell_count, e.d_nlb, e.d_err);
+                     e.d_rebuild, e.d_gcell, e.d_last, e.d_cell_count, e.d_nlb, e.d_err,
+                     e.d_part, e.nblk_part, from_partials);
   hipLaunchKernelGGL(k_nl_scan, dim3(1, e.nsys), dim3(1024), 0, e.stream, e.d_sp, e.d_rebuild,
                      e.d_cell_count, e.d_cell_start, e.d_cell_fill);
   hipLaunchKernelGGL(k_nl_scatter, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_rebuild, e.d_gcell,

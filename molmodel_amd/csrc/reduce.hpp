@@ -45,6 +45,14 @@ __device__ __forceinline__ void block_reduce(double *v, double (*red)[NV]) {
   __syncthreads();
 }
 
+// Store this block's partials only (thread 0 keeps the block's values in v).
+template <int NV, int NSUM, int TB>
+__device__ void store_partials(double *v, double *part, int blk, double (*red)[NV]) {
+  block_reduce<NV, NSUM, TB>(v, red);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < NV; c++) part[(long)blk * NV + c] = v[c];
+}
+
 // Store this block's partials and, in the last block, reduce all nblk
 // partials into out[0..NV).  Returns true in the thread that holds the
 // final values (thread 0 of the last block).
