@@ -238,10 +238,11 @@ static int compute_checked(mm_engine *m, int field_mode, int want_virial) {
     if (!field_mode) TRY(launch_finish_ex(e, FIN_FORCES, want_virial, e.d_phi));
     int f = 0;
     TRY(read_err(e, &f));
-    if (f & (ERR_NAN | ERR_CONSTRAINT)) {   // reported once, before any retry
+    if (f & (ERR_NAN | ERR_CONSTRAINT | ERR_CELL)) {   // reported once, before any retry
       TRY(clear_err(e));
-      set_last_error(f & ERR_NAN ? "non-finite position or energy during force evaluation"
-                                 : "device error flag raised during force evaluation");
+      set_last_error(f & ERR_NAN ? "non-finite (or out-of-range) position or energy during force evaluation"
+                     : f & ERR_CELL ? "positions collapsed: a neighbour-list cell is overfull"
+                                    : "device error flag raised during force evaluation");
       return MM_ERR_DEVICE;
     }
     if (f & ERR_NL_OVERFLOW) {
@@ -444,6 +445,12 @@ int mm_nlist_update(mm_nlist *nl, const double *r) {
     m->pos_dirty = 0;
     int f = 0;
     TRY(read_err(e, &f));
+    if (f & (ERR_NAN | ERR_CELL)) {
+      TRY(clear_err(e));
+      set_last_error(f & ERR_NAN ? "mm_nlist_update: non-finite (or out-of-range) position"
+                                 : "mm_nlist_update: positions collapsed, a cell is overfull");
+      return MM_ERR_DEVICE;
+    }
     if (f & ERR_NL_OVERFLOW) {
       TRY(clear_err(e));
       TRY(grow_nbr(e));
@@ -559,9 +566,10 @@ static int md_check_errors(Engine &e) {
   TRY(read_err(e, &f));
   if (f) {
     TRY(clear_err(e));   // reported once
-    set_last_error(f & ERR_NL_OVERFLOW ? "neighbour list overflow during MD"
+    set_last_error(f & ERR_NAN ? "non-finite (or out-of-range) position or energy during MD"
+                   : f & ERR_CELL ? "positions collapsed during MD: a neighbour-list cell is overfull"
                    : f & ERR_CONSTRAINT ? "constraint iteration did not converge"
-                                        : "non-finite energy during MD");
+                                        : "neighbour list overflow during MD");
     return MM_ERR_DEVICE;
   }
   return MM_OK;

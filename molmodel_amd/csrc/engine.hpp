@@ -112,7 +112,10 @@ struct Engine {
   float4 *d_bfrac = nullptr;  // ... and of d_sfrac
   // reductions
   double *d_acc = nullptr;    // NACC per system
-  double *d_part = nullptr;   // per-block partials
+  double *d_part = nullptr;   // per-block partials (reciprocal-space and exchange kernels)
+  double *d_fpart = nullptr;  // per-block partials of the finish kernel: the next
+                              // step's neighbour-list check reads them while the
+                              // reciprocal-space kernels write d_part
   int *d_done = nullptr;      // per-system block counters
   // pair-kernel work: tiles of 16 i-groups numbered over all systems
   int any_water = 0;          // some system holds only three-site waters (SysParams::water)

@@ -296,7 +296,7 @@ int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out) {
   dim3 grid((e.maxg + FIN_TB - 1) / FIN_TB, e.nsys);
 #define FIN_ARGS e.d_sp, e.d_order, e.d_gfirst, e.d_gns, e.d_gexcl, e.d_q, e.d_mass, e.d_pf, \
     e.d_gacc, e.d_kres, e.d_r, e.d_v, e.d_f, phi_out, e.d_last, e.nsite, want_virial, e.bond,  \
-    e.angle * (M_PI / 180.0), e.tol, e.d_part, e.d_done, e.nblk_part, e.d_acc, e.d_rebuild,   \
+    e.angle * (M_PI / 180.0), e.tol, e.d_fpart, e.d_done, e.nblk_part, e.d_acc, e.d_rebuild,  \
     e.d_err, e.d_gpos, e.d_spk, e.d_ginfo
 #define FIN_CASE(M)                                                                            \
   if (want_virial)                                                                             \

@@ -32,6 +32,7 @@ enum : int {
   ERR_CONSTRAINT = 4,        // SHAKE did not converge
   ERR_NAN = 8,               // non-finite energy
   ERR_BIN_OVERFLOW = 16,     // P3M column bin overflow
+  ERR_CELL = 32,             // a neighbour-list cell holds more groups than its sort takes
 };
 
 struct SysParams {

@@ -79,7 +79,10 @@ __global__ void __launch_bounds__(256) k_nl_assign(
   const double x = r[site], y = r[nsite_total + site], z = r[2 * nsite_total + site];
   const int n = P.ncell;
   int c = P.cell0;
-  if (fabs(x) <= 1e300 && fabs(y) <= 1e300 && fabs(z) <= 1e300) {
+  // cell indices stay far inside int range; a position beyond (or NaN) is
+  // an error
+  const double lim = 1e9 * P.lc;
+  if (fabs(x) <= lim && fabs(y) <= lim && fabs(z) <= lim) {
     const int ix = (int)floor(x / P.lc + 0.5);
     const int iy = (int)floor(y / P.lc + 0.5);
     const int iz = (int)floor(z / P.lc + 0.5);
@@ -119,12 +122,16 @@ __global__ void k_nl_scatter(const SysParams *__restrict__ sp, const int *__rest
   order[cell_start[c] + atomicAdd(&cell_fill[c], 1)] = g;
 }
 
+// groups one cell may hold: far above any physical density (cells hold
+// about two groups on average), low enough that the serial sort stays short
+constexpr int NL_SORT_MAX = 512;
+
 // ascending group id within each cell, so the sorted order is deterministic;
 // then the first-site positions in sorted order (the pair kernel's gpos)
 __global__ void k_nl_sortcell(const SysParams *__restrict__ sp, const int *__restrict__ rebuild,
                               const int *__restrict__ cell_start, int *__restrict__ order,
                               const double *__restrict__ last, double4 *__restrict__ gpos,
-                              const int *__restrict__ err) {
+                              int *__restrict__ err) {
   const int s = blockIdx.y;
   if (!rebuild[s]) return;
   // after a non-finite position (its group parked in cell 0) skip the sort,
@@ -135,6 +142,10 @@ __global__ void k_nl_sortcell(const SysParams *__restrict__ sp, const int *__res
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.ncell * P.ncell * P.ncell) return;
   const int b = cell_start[P.cell0 + c], e = cell_start[P.cell0 + c + 1];
+  if (e - b > NL_SORT_MAX) {   // collapsed positions: reported, not sorted
+    atomicOr(err, ERR_CELL);
+    return;
+  }
   for (int i = b + 1; i < e; i++) {
     const int v = order[i];
     int j = i - 1;
@@ -174,9 +185,17 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
   // grid-stride over sorted groups: a small grid keeps the launch cheap on
   // the many steps that do not rebuild
   const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
+  // after a non-finite or collapsed position the cells are meaningless (all
+  // groups may share one): leave the rows empty
+  const bool bad =
+      (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & (ERR_NAN | ERR_CELL)) != 0;
   for (int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); wave < P.ngroup;
        wave += nwave) {
     const int gs = P.group0 + wave;
+    if (bad) {   // the error is reported; the rows stay empty
+      if (lane == 0) nnb[gs] = 0;
+      continue;
+    }
     const int g = order[gs];
     const int n = P.ncell;
     const int A = gcell[g] - P.cell0;
@@ -356,7 +375,7 @@ int launch_nl_build(Engine &e, int force, int from_partials) {
   const double *r = e.d_r;
   hipLaunchKernelGGL(k_nl_assign, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_gfirst, r, e.nsite,
                      e.d_rebuild, e.d_gcell, e.d_last, e.d_cell_count, e.d_nlb, e.d_err,
-                     e.d_part, e.nblk_part, from_partials);
+                     e.d_fpart, e.nblk_part, from_partials);
   hipLaunchKernelGGL(k_nl_scan, dim3(1, e.nsys), dim3(1024), 0, e.stream, e.d_sp, e.d_rebuild,
                      e.d_cell_count, e.d_cell_start, e.d_cell_fill);
   hipLaunchKernelGGL(k_nl_scatter, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_rebuild, e.d_gcell,

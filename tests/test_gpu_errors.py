@@ -54,3 +54,23 @@ def test_all_positions_non_finite_is_cheap(gpu_lib):
     e.set_positions(s.r)
     e.compute()
     assert np.isfinite(e.forces()).all()
+
+
+def test_collapsed_positions_reported_then_recovered(gpu_lib):
+    """Every water moved onto one point (finite positions, all groups in one
+    neighbour-list cell): the build refuses to sort an overfull cell and
+    reports it instead of spending a serial sort on 4096 groups."""
+    s, p = W.config("water12k_p3m")
+    e = Engine(s, p)
+    e.set_positions(s.r)
+    e.compute()
+    f0 = e.forces()
+    first = s.r[s.group[:, 0]]
+    r = s.r - np.repeat(first, 3, axis=0) + np.array([10.0, 10.0, 10.0])
+    e.set_positions(r)
+    print("[errors] all groups on one point", flush=True)
+    with pytest.raises(MMError, match="collapsed"):
+        e.compute()
+    e.set_positions(s.r)
+    e.compute()
+    assert np.array_equal(e.forces(), f0)
