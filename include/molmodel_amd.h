@@ -40,7 +40,8 @@ extern "C" {
 /* device error flags reported by mm_errors() */
 #define MM_FLAG_NL_OVERFLOW 1
 #define MM_FLAG_CONSTRAINT 4
-#define MM_FLAG_NAN 8
+#define MM_FLAG_NAN 8          /* non-finite energy, or a non-finite / out-of-range position */
+#define MM_FLAG_CELL 32        /* positions collapsed: a neighbour-list cell held > 512 groups */
 
 #define MM_SITE_HAS_Q 1      /* rspace.h:7 */
 #define MM_SITE_HAS_LJ 2     /* rspace.h:8 */
