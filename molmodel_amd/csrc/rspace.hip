@@ -333,7 +333,7 @@ __device__ __forceinline__ void pair_tile(
       // smoothing force on the first site (rspace.c:479-489)
       const float fs = dsmf * tmps;
       const float sx = fs * Dx, sy = fs * Dy, sz = fs * Dz;
-      double Fx = sx, Fy = sy, Fz = sz;
+      double Fx = 0, Fy = 0, Fz = 0;
 #pragma unroll
       for (int k = 0; k < NSI; k++) {
         ap[k] += (double)ph[k];
@@ -341,14 +341,15 @@ __device__ __forceinline__ void pair_tile(
           ax[k] += (double)fx[k]; ay[k] += (double)fy[k]; az[k] += (double)fz[k];
           lx[k] += (double)gx[k]; ly[k] += (double)gy[k]; lz[k] += (double)gz[k];
         } else {
+          // the smoothing force joins the first site's FP32 sum
           const float qk = oi[k].w;
-          const float tx = qk * fx[k] + gx[k], ty = qk * fy[k] + gy[k], tz = qk * fz[k] + gz[k];
+          float tx = qk * fx[k] + gx[k], ty = qk * fy[k] + gy[k], tz = qk * fz[k] + gz[k];
+          if (k == 0) { tx += sx; ty += sy; tz += sz; }
           ax[k] += (double)tx; ay[k] += (double)ty; az[k] += (double)tz;
           if (VIRIAL) { Fx += tx; Fy += ty; Fz += tz; }
         }
       }
       if (FIELD) { lx[0] += (double)sx; ly[0] += (double)sy; lz[0] += (double)sz; }
-      else { ax[0] += (double)sx; ay[0] += (double)sy; az[0] += (double)sz; }
       ulj += (double)uu;
       if (VIRIAL) {   // 1/2 D (x) F_gh per directed pair
         const double dx = Dx, dy = Dy, dz = Dz;
