@@ -203,7 +203,7 @@ enum : int {   // launch_finish modes
 };
 // finish kernel blocks and the layout of their partial sums without the
 // virial: elec, lj, kin, largest and second largest displacement
-constexpr int FIN_TB = 256;
+constexpr int FIN_TB = 128;
 constexpr int FIN_NPART = 5;
 constexpr int FIN_TOP = 3;
 
