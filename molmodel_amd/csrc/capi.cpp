@@ -189,9 +189,10 @@ static int grow_nbr(Engine &e) {
   return 0;
 }
 
-// nl_from_partials: the last finish left per-block partial sums (FIN_LAZY)
+// nl_from_partials: the last finish left per-block partial sums (FIN_LAZY);
+// mesh_energy 0: an inner MD step, whose energies are not formed
 static int enqueue_forces(mm_engine *m, int field_mode, int want_virial,
-                          int nl_from_partials = 0) {
+                          int nl_from_partials = 0, int mesh_energy = 1) {
   Engine &e = m->e;
   if (e.no_rspace) {
     TRY(launch_kspace(e, want_virial));
@@ -210,7 +211,7 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial,
     kst = e.stream2;
   }
   TRY(launch_kspace(e, want_virial, kst));
-  TRY(launch_p3m(e, want_virial, kst));
+  TRY(launch_p3m(e, want_virial, kst, mesh_energy || want_virial));
   if (e.external_nl) {
     e.first_eval = 0;
   } else if (e.first_eval) {
@@ -525,7 +526,7 @@ int mm_md_setup(mm_engine *m, double dt, double bond, double angle_deg, double t
 // one MD step after a finish; inner steps leave partial sums (FIN_LAZY),
 // which the next step's neighbour-list check reduces itself
 static int enqueue_step(mm_engine *m, int mode) {
-  TRY(enqueue_forces(m, 0, 0, 1));
+  TRY(enqueue_forces(m, 0, 0, 1, (mode & FIN_LAZY) ? 0 : 1));
   TRY(launch_finish_ex(m->e, mode, 0, m->e.d_phi));
   return 0;
 }

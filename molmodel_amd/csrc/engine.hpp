@@ -175,7 +175,8 @@ int pair_tiles(int ngroup);
 constexpr int P3M_FFT_LINES = 16;
 
 int launch_kspace(Engine &e, int want_virial, hipStream_t st);
-int launch_p3m(Engine &e, int want_virial, hipStream_t st);
+// energies 0: the mesh energy (a diagnostic total) is not formed
+int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies = 1);
 inline int launch_kspace(Engine &e, int want_virial) { return launch_kspace(e, want_virial, e.stream); }
 inline int launch_p3m(Engine &e, int want_virial) { return launch_p3m(e, want_virial, e.stream); }
 int launch_finish(Engine &e, int mode);

@@ -403,8 +403,9 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
 
 // x lines: forward FFT, multiply by the influence function, inverse FFT.
 // Also the mesh energy 1/2 sum G |rho_k|^2 and its virial
-// W_ab = sum e_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 eta^2))).
-template <bool VIR, bool POW2>
+// W_ab = sum e_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 eta^2))); ENER false:
+// the convolution alone (inner MD steps, whose energies are not formed).
+template <bool VIR, bool POW2, bool ENER = true>
 __global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__restrict__ sp,
                                                        const float2 *__restrict__ twid_all,
                                                        const float *__restrict__ green,
@@ -450,7 +451,7 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__res
     for (int t = tid; t < nx; t += FFT_T) {
       const float2 v = rl[t];
       const float gg = g[t];
-      const double ek = 0.5 * mult * (double)gg * ((double)v.x * v.x + (double)v.y * v.y);
+      const double ek = ENER ? 0.5 * mult * (double)gg * ((double)v.x * v.x + (double)v.y * v.y) : 0.0;
       e += ek;
       if (VIR && gg != 0.f) {
         const int jx = t > nx / 2 ? t - nx : t;
@@ -470,6 +471,7 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__res
     float2 *rl = res + l * FFT_MAXN;
     for (int t = tid; t < nx; t += FFT_T) src[t * xstride] = rl[t];
   }
+  if constexpr (!ENER) return;
   // deterministic block partials, reduced by the last block
   double vals[NV];
   vals[0] = e;
@@ -539,7 +541,8 @@ __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict_
 
 // forward z, y passes; x pass with the influence function; inverse y, z
 template <bool POW2>
-static void launch_fft(Engine &e, int want_virial, hipStream_t st, int maxx, int maxy, int maxz) {
+static void launch_fft(Engine &e, int want_virial, int energies, hipStream_t st, int maxx,
+                       int maxy, int maxz) {
   hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 0);
   hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
@@ -548,9 +551,13 @@ static void launch_fft(Engine &e, int want_virial, hipStream_t st, int maxx, int
     hipLaunchKernelGGL((k_fft_xgreen<true, POW2>), dim3(maxx, e.nsys), dim3(FFT_TB), 0, st,
                        e.d_sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
                        e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
-  else
+  else if (energies)
     hipLaunchKernelGGL((k_fft_xgreen<false, POW2>), dim3(maxx, e.nsys), dim3(FFT_TB), 0, st,
                        e.d_sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
+                       e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
+  else
+    hipLaunchKernelGGL((k_fft_xgreen<false, POW2, false>), dim3(maxx, e.nsys), dim3(FFT_TB), 0,
+                       st, e.d_sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
                        e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
   hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 3);
@@ -558,7 +565,7 @@ static void launch_fft(Engine &e, int want_virial, hipStream_t st, int maxx, int
                      e.d_twid, e.d_mesh, e.d_cmesh, 4);
 }
 
-int launch_p3m(Engine &e, int want_virial, hipStream_t st) {
+int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies) {
   int maxn = 0, maxbins = 0, maxz = 0, maxy = 0, maxx = 0;
   for (auto &p : e.hsp)
     if (p.kmode == 1) {
@@ -593,8 +600,8 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st) {
   for (auto &p : e.hsp)
     if (p.kmode == 1)
       for (int n : {p.nx, p.ny, p.nz}) pow2 &= (n & (n - 1)) == 0;
-  if (pow2) launch_fft<true>(e, want_virial, st, maxx, maxy, maxz);
-  else launch_fft<false>(e, want_virial, st, maxx, maxy, maxz);
+  if (pow2) launch_fft<true>(e, want_virial, energies, st, maxx, maxy, maxz);
+  else launch_fft<false>(e, want_virial, energies, st, maxx, maxy, maxz);
 #define GATHER(PO)                                                                        \
   if (used[PO])                                                                           \
     hipLaunchKernelGGL(k_p3m_gather<PO>, gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid,     \
