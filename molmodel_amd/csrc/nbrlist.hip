@@ -182,8 +182,10 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int *cs = c_start[wv], *co = c_off[wv], *cc = c_cell[wv];
   const unsigned long long below = (1ull << lane) - 1ull;
-  // grid-stride over sorted groups: a small grid keeps the launch cheap on
-  // the many steps that do not rebuild
+  // one wavefront per sorted group (the loop strides only if the grid is
+  // smaller); more waves in flight hide the candidate loads of a rebuild
+  // better than a grid-stride over a small grid (measured, and the empty
+  // launches of the steps that do not rebuild cost no more)
   const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
   // after a non-finite or collapsed position the cells are meaningless (all
   // groups may share one): leave the rows empty
@@ -385,7 +387,7 @@ int launch_nl_build(Engine &e, int force, int from_partials) {
   hipLaunchKernelGGL(k_nl_sortcell, dim3((maxc + tb - 1) / tb, e.nsys), dim3(tb), 0, e.stream,
                      e.d_sp, e.d_rebuild, e.d_cell_start, e.d_order, e.d_last, e.d_gpos,
                      e.d_err);
-  hipLaunchKernelGGL(k_nl_pairs, dim3(std::min((e.maxg + 3) / 4, 1024), e.nsys), dim3(256), 0,
+  hipLaunchKernelGGL(k_nl_pairs, dim3((e.maxg + 3) / 4, e.nsys), dim3(256), 0,
                      e.stream, e.d_sp, e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start,
                      e.d_sten, e.nbcap, e.d_nbr, e.d_nnb, e.d_err, e.d_r, e.nsite, e.d_q,
                      e.d_gfirst, e.d_gns, e.d_ginfo, e.d_gpos, e.d_spk, e.d_gmeta);
