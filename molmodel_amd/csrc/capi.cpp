@@ -239,10 +239,11 @@ static int compute_checked(mm_engine *m, int field_mode, int want_virial) {
     if (!field_mode) TRY(launch_finish_ex(e, FIN_FORCES, want_virial, e.d_phi));
     int f = 0;
     TRY(read_err(e, &f));
-    if (f & (ERR_NAN | ERR_CONSTRAINT | ERR_CELL)) {   // reported once, before any retry
+    if (f & (ERR_NAN | ERR_CONSTRAINT | ERR_CELL | ERR_BIN_OVERFLOW)) {   // reported once, before any retry
       TRY(clear_err(e));
       set_last_error(f & ERR_NAN ? "non-finite (or out-of-range) position or energy during force evaluation"
                      : f & ERR_CELL ? "positions collapsed: a neighbour-list cell is overfull"
+                     : f & ERR_BIN_OVERFLOW ? "a P3M mesh column bin overflowed (sites far denser than the box average)"
                                     : "device error flag raised during force evaluation");
       return MM_ERR_DEVICE;
     }
@@ -569,6 +570,7 @@ static int md_check_errors(Engine &e) {
     TRY(clear_err(e));   // reported once
     set_last_error(f & ERR_NAN ? "non-finite (or out-of-range) position or energy during MD"
                    : f & ERR_CELL ? "positions collapsed during MD: a neighbour-list cell is overfull"
+                   : f & ERR_BIN_OVERFLOW ? "a P3M mesh column bin overflowed during MD"
                    : f & ERR_CONSTRAINT ? "constraint iteration did not converge"
                                         : "neighbour list overflow during MD");
     return MM_ERR_DEVICE;

@@ -566,11 +566,12 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   MM_TRY(dalloc(&e.d_twid, twid.size())); MM_TRY(dput(e.d_twid, twid));
   MM_TRY(dalloc(&e.d_sgrid, N));
   MM_TRY(dalloc(&e.d_sfrac, N));
-  MM_TRY(dalloc(&e.d_bin_count, e.nbin_total + 1));
-  MM_TRY(dalloc(&e.d_bin_start, e.nbin_total + 1));
   MM_TRY(dalloc(&e.d_bin_fill, e.nbin_total + 1));
-  MM_TRY(dalloc(&e.d_bgrid, N));
-  MM_TRY(dalloc(&e.d_bfrac, N));
+  e.bcap = 32;
+  for (auto &P : e.hsp)
+    if (P.kmode == 1) e.bcap = std::max(e.bcap, p3m_bin_capacity(P.nsite, P.nbx * P.nby));
+  MM_TRY(dalloc(&e.d_bgrid, (size_t)std::max(e.nbin_total, 1) * e.bcap));
+  MM_TRY(dalloc(&e.d_bfrac, (size_t)std::max(e.nbin_total, 1) * e.bcap));
   MM_TRY(dalloc(&e.d_acc, (size_t)e.nsys * NACC));
   MM_TRY(dalloc(&e.d_part, (size_t)e.nsys * e.nblk_part * 16));
   MM_TRY(dalloc(&e.d_fpart, (size_t)e.nsys * e.nblk_part * 16));
@@ -692,7 +693,7 @@ void engine_destroy(Engine &e) {
                   e.d_cell_fill, e.d_order, e.d_last, e.d_sten, e.d_nbr, e.d_nnb, e.d_rebuild,
                   e.d_err, e.d_gpos, e.d_spk, e.d_gmeta, e.d_pf, e.d_plj, e.d_gacc, e.d_tab,
                   e.d_ljt, e.d_kres, e.d_kvec, e.d_ksc, e.d_mesh, e.d_cmesh, e.d_green, e.d_twid,
-                  e.d_sgrid, e.d_sfrac, e.d_bin_count, e.d_bin_start, e.d_bin_fill,
+                  e.d_sgrid, e.d_sfrac, e.d_bin_fill,
                   e.d_bgrid, e.d_bfrac, e.d_acc, e.d_part, e.d_fpart, e.d_done, e.d_phi, e.d_scratch,
                   e.d_nlb, e.d_ladder, e.d_kT, e.d_tail, e.d_rtmp, e.d_mom, e.d_perm,
                   e.d_remd_stats, e.d_tile_start};

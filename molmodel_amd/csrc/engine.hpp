@@ -107,7 +107,7 @@ struct Engine {
   int4 *d_sgrid = nullptr;    // per site: jx jy jz bin
   float4 *d_sfrac = nullptr;  // per site: sx sy sz q
   int nbin_total = 0;
-  int *d_bin_count = nullptr, *d_bin_start = nullptr, *d_bin_fill = nullptr;
+  int *d_bin_fill = nullptr;   // sites per P3M column bin (cleared by the gather)
   int4 *d_bgrid = nullptr;    // bin-sorted copies of d_sgrid (w = site index) ...
   float4 *d_bfrac = nullptr;  // ... and of d_sfrac
   // reductions
@@ -118,6 +118,7 @@ struct Engine {
                               // reciprocal-space kernels write d_part
   int *d_done = nullptr;      // per-system block counters
   // pair-kernel work: tiles of 16 i-groups numbered over all systems
+  int bcap = 0;               // slots per P3M column bin (bgrid/bfrac hold nbin_total * bcap)
   int any_water = 0;          // some system holds only three-site waters (SysParams::water)
   int ntile = 0;
   int pair_blocks = 0;        // persistent grid (a multiple of 8)

@@ -95,6 +95,13 @@ inline bool choose_spread_blocks(int nx, int ny, int nz, int maxpts, int maxbins
   return bx * by * nz <= maxpts && (nx / bx) * (ny / by) <= maxbins;
 }
 
+// slots per P3M column bin: four times the mean occupancy plus a margin,
+// rounded to 32 (an overfull bin raises ERR_BIN_OVERFLOW)
+inline int p3m_bin_capacity(int nsite, int nbins) {
+  const int mean = (nsite + nbins - 1) / nbins;
+  return (4 * mean + 64 + 31) / 32 * 32;
+}
+
 __host__ __device__ inline void canonical_pair(int &i, int &j) {   // pbin.c:6
   if (((i + j) % 2) ^ (i < j)) { int t = i; i = j; j = t; }
 }

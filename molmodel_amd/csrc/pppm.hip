@@ -67,67 +67,22 @@ __device__ __forceinline__ void bspline_w(float s, float *w, float *ndw) {
 // wrap j + off into [0, n) for |off| < n (no integer division)
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// Grid coordinates of each site (pppm.c:540-557) and its copy in the slots
+// of its mesh column bin (bx x by columns).  A block ranks its sites per bin
+// in LDS and claims each bin's slots with one global atomic; the order
+// inside a bin varies from run to run, and the spreading's 64-bit
+// fixed-point sums do not depend on it.  The gather clears the counters.
 constexpr int MAXBINS = 4096;
-
-__global__ void __launch_bounds__(256) k_p3m_grid(const SysParams *__restrict__ sp,
-                                                  const double *__restrict__ r,
-                                                  const double *__restrict__ q, int N,
-                                                  int4 *__restrict__ sgrid,
-                                                  float4 *__restrict__ sfrac,
-                                                  int *__restrict__ bin_count) {
-  __shared__ int hist[MAXBINS];
-  const int s = blockIdx.y;
-  const SysParams &P = sp[s];
-  if (P.kmode != 1) return;
-  if ((int)(blockIdx.x * blockDim.x) >= P.nsite) return;
-  const int nb = P.nbx * P.nby;
-  for (int t = threadIdx.x; t < nb; t += blockDim.x) hist[t] = 0;
-  __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < P.nsite) {
-    const int si = P.site0 + i;
-    const double x = r[si], y = r[N + si], z = r[2 * N + si];
-    const double *T = P.T;
-    double sx = T[0] * x + T[3] * y + T[6] * z;
-    double sy = T[1] * x + T[4] * y + T[7] * z;
-    double sz = T[2] * x + T[5] * y + T[8] * z;
-    int jx, jy, jz;
-    if (P.order & 1) {
-      jx = (int)floor(sx + 0.5); jy = (int)floor(sy + 0.5); jz = (int)floor(sz + 0.5);
-      sx -= jx; sy -= jy; sz -= jz;
-    } else {
-      jx = (int)floor(sx); jy = (int)floor(sy); jz = (int)floor(sz);
-      sx -= jx + 0.5; sy -= jy + 0.5; sz -= jz + 0.5;
-    }
-    jx = mod_nonneg(jx, P.nx); jy = mod_nonneg(jy, P.ny); jz = mod_nonneg(jz, P.nz);
-    const int lb = (jx / P.bx) * P.nby + jy / P.by;
-    sgrid[si] = make_int4(jx, jy, jz, P.bin0 + lb);
-    sfrac[si] = make_float4((float)sx, (float)sy, (float)sz, (float)q[si]);
-    atomicAdd(&hist[lb], 1);
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < nb; t += blockDim.x)
-    if (hist[t]) atomicAdd(&bin_count[P.bin0 + t], hist[t]);
-}
-
-__global__ void k_p3m_binscan(const SysParams *__restrict__ sp, int *__restrict__ bin_count,
-                              int *__restrict__ bin_start, int *__restrict__ bin_fill) {
-  const int s = blockIdx.y;
-  const SysParams &P = sp[s];
-  if (P.kmode != 1) return;
-  block_scan_clear(bin_count + P.bin0, bin_start + P.bin0, bin_fill + P.bin0, P.nbx * P.nby,
-                   P.site0);
-}
-
-__global__ void __launch_bounds__(256) k_p3m_binscatter(const SysParams *__restrict__ sp,
-                                                        const int4 *__restrict__ sgrid,
-                                                        const float4 *__restrict__ sfrac,
-                                                        const int *__restrict__ bin_start,
-                                                        int *__restrict__ bin_fill,
-                                                        int4 *__restrict__ bgrid,
-                                                        float4 *__restrict__ bfrac) {
+__global__ void __launch_bounds__(256) k_p3m_bin(const SysParams *__restrict__ sp,
+                                                 const double *__restrict__ r,
+                                                 const double *__restrict__ q, int N,
+                                                 int4 *__restrict__ sgrid,
+                                                 float4 *__restrict__ sfrac,
+                                                 int *__restrict__ fill, int bcap,
+                                                 int4 *__restrict__ bgrid,
+                                                 float4 *__restrict__ bfrac,
+                                                 int *__restrict__ err) {
   __shared__ int cnt[MAXBINS];
-  __shared__ int base[MAXBINS];
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
   if (P.kmode != 1) return;
@@ -136,22 +91,43 @@ __global__ void __launch_bounds__(256) k_p3m_binscatter(const SysParams *__restr
   for (int t = threadIdx.x; t < nb; t += blockDim.x) cnt[t] = 0;
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int lb = -1, rank = 0;
-  int4 g;
-  if (i < P.nsite) {
-    g = sgrid[P.site0 + i];
-    lb = g.w - P.bin0;
+  const bool on = i < P.nsite;
+  const int si = P.site0 + (on ? i : 0);
+  const double x = r[si], y = r[N + si], z = r[2 * N + si];
+  const double *T = P.T;
+  double sx = T[0] * x + T[3] * y + T[6] * z;
+  double sy = T[1] * x + T[4] * y + T[7] * z;
+  double sz = T[2] * x + T[5] * y + T[8] * z;
+  int jx, jy, jz;
+  if (P.order & 1) {
+    jx = (int)floor(sx + 0.5); jy = (int)floor(sy + 0.5); jz = (int)floor(sz + 0.5);
+    sx -= jx; sy -= jy; sz -= jz;
+  } else {
+    jx = (int)floor(sx); jy = (int)floor(sy); jz = (int)floor(sz);
+    sx -= jx + 0.5; sy -= jy + 0.5; sz -= jz + 0.5;
+  }
+  jx = mod_nonneg(jx, P.nx); jy = mod_nonneg(jy, P.ny); jz = mod_nonneg(jz, P.nz);
+  const int lb = (jx / P.bx) * P.nby + jy / P.by;
+  const int b = P.bin0 + lb;
+  const float4 f = make_float4((float)sx, (float)sy, (float)sz, (float)q[si]);
+  int rank = 0;
+  if (on) {
+    sgrid[si] = make_int4(jx, jy, jz, b);
+    sfrac[si] = f;
     rank = atomicAdd(&cnt[lb], 1);
   }
   __syncthreads();
+  // one claim per bin this block touches; the base replaces the count
   for (int t = threadIdx.x; t < nb; t += blockDim.x)
-    if (cnt[t]) base[t] = bin_start[P.bin0 + t] + atomicAdd(&bin_fill[P.bin0 + t], cnt[t]);
+    if (cnt[t]) cnt[t] = atomicAdd(&fill[P.bin0 + t], cnt[t]);
   __syncthreads();
-  if (lb >= 0) {
-    // copies in bin order, so the spreading blocks read them contiguously
-    const int pos = base[lb] + rank;
-    bgrid[pos] = make_int4(g.x, g.y, g.z, P.site0 + i);
-    bfrac[pos] = sfrac[P.site0 + i];
+  if (!on) return;
+  const int slot = cnt[lb] + rank;
+  if (slot < bcap) {
+    bgrid[(long)b * bcap + slot] = make_int4(jx, jy, jz, si);
+    bfrac[(long)b * bcap + slot] = f;
+  } else {
+    atomicOr(err, ERR_BIN_OVERFLOW);   // reported; the site is left out of the mesh
   }
 }
 
@@ -163,7 +139,7 @@ constexpr int SPREAD_MAXPTS = 4096;
 
 template <int P>
 __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict__ sp,
-                                                    const int *__restrict__ bin_start,
+                                                    const int *__restrict__ fill, int bcap,
                                                     const int4 *__restrict__ bgrid,
                                                     const float4 *__restrict__ bfrac,
                                                     float *__restrict__ mesh) {
@@ -196,8 +172,8 @@ __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict_
     if (t < nbin) {
       const int ccx = mod_nonneg(xlo + t / ncy, Q.nbx), ccy = mod_nonneg(ylo + t % ncy, Q.nby);
       const int b = Q.bin0 + ccx * Q.nby + ccy;
-      sb_start[t] = bin_start[b];
-      cnt = bin_start[b + 1] - sb_start[t];
+      sb_start[t] = b * bcap;
+      cnt = min(fill[b], bcap);
     }
     int incl = cnt;   // inclusive scan over the wave
 #pragma unroll
@@ -489,10 +465,13 @@ __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict_
                                                     const int4 *__restrict__ sgrid,
                                                     const float4 *__restrict__ sfrac,
                                                     const float *__restrict__ mesh,
-                                                    double *__restrict__ kres) {
+                                                    double *__restrict__ kres,
+                                                    int *__restrict__ fill) {
   const int s = blockIdx.y;
   const SysParams &Q = sp[s];
   if (Q.kmode != 1 || Q.order != P) return;
+  if (blockIdx.x == 0)   // the spreading has read the bin counters: clear them
+    for (int t = threadIdx.x; t < Q.nbx * Q.nby; t += blockDim.x) fill[Q.bin0 + t] = 0;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Q.nsite) return;
   const int si = Q.site0 + i;
@@ -579,19 +558,15 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies) {
   if (maxn == 0) return 0;
   const int tb = 256;
   dim3 gs((maxn + tb - 1) / tb, e.nsys);
-  hipLaunchKernelGGL(k_p3m_grid, gs, dim3(tb), 0, st, e.d_sp, e.d_r, e.d_q, e.nsite,
-                     e.d_sgrid, e.d_sfrac, e.d_bin_count);
-  hipLaunchKernelGGL(k_p3m_binscan, dim3(1, e.nsys), dim3(1024), 0, st, e.d_sp,
-                     e.d_bin_count, e.d_bin_start, e.d_bin_fill);
-  hipLaunchKernelGGL(k_p3m_binscatter, gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid,
-                     e.d_sfrac, e.d_bin_start, e.d_bin_fill, e.d_bgrid, e.d_bfrac);
+  hipLaunchKernelGGL(k_p3m_bin, gs, dim3(tb), 0, st, e.d_sp, e.d_r, e.d_q, e.nsite,
+                     e.d_sgrid, e.d_sfrac, e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_err);
   bool used[MAXORDER + 1] = {false};
   for (auto &p : e.hsp)
     if (p.kmode == 1) used[p.order] = true;
 #define SPREAD(PO)                                                                          \
   if (used[PO])                                                                             \
     hipLaunchKernelGGL(k_p3m_spread<PO>, dim3(maxbins, e.nsys), dim3(256), 0, st,     \
-                       e.d_sp, e.d_bin_start, e.d_bgrid, e.d_bfrac, e.d_mesh);
+                       e.d_sp, e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_mesh);
   SPREAD(1) SPREAD(2) SPREAD(3) SPREAD(4) SPREAD(5) SPREAD(6) SPREAD(7)
 #undef SPREAD
   // meshes whose sides are all powers of two take FFT kernels built with
@@ -605,7 +580,7 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies) {
 #define GATHER(PO)                                                                        \
   if (used[PO])                                                                           \
     hipLaunchKernelGGL(k_p3m_gather<PO>, gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid,     \
-                       e.d_sfrac, e.d_mesh, e.d_kres);
+                       e.d_sfrac, e.d_mesh, e.d_kres, e.d_bin_fill);
   GATHER(1) GATHER(2) GATHER(3) GATHER(4) GATHER(5) GATHER(6) GATHER(7)
 #undef GATHER
   MM_HIP(hipGetLastError());

@@ -605,13 +605,13 @@ extern "C" p3m_t p3m_new(int nsite, const double ptv[9], double grid_spacing, in
   HIPCHK(hipMalloc((void **)&e.d_sgrid, sizeof(int4) * N));
   HIPCHK(hipMalloc((void **)&e.d_sfrac, sizeof(float4) * N));
   const int nb = P.nbx * P.nby + 1;
-  HIPCHK(hipMalloc((void **)&e.d_bin_count, sizeof(int) * nb));
-  HIPCHK(hipMalloc((void **)&e.d_bin_start, sizeof(int) * nb));
   HIPCHK(hipMalloc((void **)&e.d_bin_fill, sizeof(int) * nb));
-  HIPCHK(hipMemset(e.d_bin_count, 0, sizeof(int) * nb));
   HIPCHK(hipMemset(e.d_bin_fill, 0, sizeof(int) * nb));
-  HIPCHK(hipMalloc((void **)&e.d_bgrid, sizeof(int4) * N));
-  HIPCHK(hipMalloc((void **)&e.d_bfrac, sizeof(float4) * N));
+  e.bcap = p3m_bin_capacity(N, P.nbx * P.nby);
+  HIPCHK(hipMalloc((void **)&e.d_bgrid, sizeof(int4) * (size_t)(nb - 1) * e.bcap));
+  HIPCHK(hipMalloc((void **)&e.d_bfrac, sizeof(float4) * (size_t)(nb - 1) * e.bcap));
+  HIPCHK(hipMalloc((void **)&e.d_err, sizeof(int)));   // the binning kernel flags overfull bins
+  HIPCHK(hipMemset(e.d_err, 0, sizeof(int)));
   HIPCHK(hipMalloc((void **)&e.d_part, sizeof(double) * 16 * e.nblk_part));
   HIPCHK(hipMalloc((void **)&e.d_done, sizeof(int) * 4));
   HIPCHK(hipMemset(e.d_done, 0, sizeof(int) * 4));
@@ -667,6 +667,12 @@ extern "C" void p3m_field(p3m_t p, const double *r, const double *q, double *phi
   HIPCHK(hipMemcpy(e.d_q, q, sizeof(double) * N, hipMemcpyHostToDevice));
   if (launch_p3m(e, 0)) die_msg("p3m_field", last_error());
   HIPCHK(hipStreamSynchronize(e.stream));
+  int flags = 0;
+  HIPCHK(hipMemcpy(&flags, e.d_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (flags & ERR_BIN_OVERFLOW) {
+    HIPCHK(hipMemset(e.d_err, 0, sizeof(int)));
+    die_msg("p3m_field", "a mesh column bin overflowed (sites far denser than the box average)");
+  }
   std::vector<double> out(4 * (size_t)N);
   HIPCHK(hipMemcpy(out.data(), e.d_kres, sizeof(double) * out.size(), hipMemcpyDeviceToHost));
   for (int i = 0; i < N; i++) {
@@ -700,8 +706,8 @@ extern "C" void p3m_delete(p3m_t p) {
   if (!p) return;
   Engine &e = p->e;
   void *ptrs[] = {e.d_sp, e.d_r, e.d_q, e.d_kres, e.d_acc, e.d_mesh, e.d_cmesh, e.d_green,
-                  e.d_sgrid, e.d_sfrac, e.d_bin_count, e.d_bin_start, e.d_bin_fill, e.d_bgrid, e.d_bfrac,
-                  e.d_part, e.d_done, e.d_twid};
+                  e.d_sgrid, e.d_sfrac, e.d_bin_fill, e.d_bgrid, e.d_bfrac,
+                  e.d_part, e.d_done, e.d_twid, e.d_err};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (e.stream) hipStreamDestroy(e.stream);
