@@ -542,6 +542,12 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   MM_TRY(dalloc(&e.d_cell_start, e.ncell_total + 1));
   MM_TRY(dalloc(&e.d_cell_fill, e.ncell_total + 1));
   MM_TRY(dalloc(&e.d_order, G));
+  {
+    std::vector<int4> sl(std::max(G, 1), make_int4(0, 0, 0, 0));
+    for (int g = 0; g < G; g++) sl[g] = make_int4(g, gfirst[g], gns[g], gexcl[g]);
+    MM_TRY(dalloc(&e.d_gslot, sl.size()));
+    MM_TRY(dput(e.d_gslot, sl));
+  }
   MM_TRY(dalloc(&e.d_last, 3 * (size_t)G));
   MM_TRY(dalloc(&e.d_sten, sten.size())); MM_TRY(dput(e.d_sten, sten));
   MM_TRY(dalloc(&e.d_nbr, (size_t)G * e.nbcap));
@@ -690,7 +696,7 @@ void engine_destroy(Engine &e) {
   if (e.graph) hipGraphDestroy(e.graph);
   void *ptrs[] = {e.d_sp, e.d_gfirst, e.d_gns, e.d_gsys, e.d_gexcl, e.d_ginfo, e.d_q, e.d_mass,
                   e.d_r, e.d_v, e.d_f, e.d_r0, e.d_gcell, e.d_cell_count, e.d_cell_start,
-                  e.d_cell_fill, e.d_order, e.d_last, e.d_sten, e.d_nbr, e.d_nnb, e.d_rebuild,
+                  e.d_cell_fill, e.d_order, e.d_gslot, e.d_last, e.d_sten, e.d_nbr, e.d_nnb, e.d_rebuild,
                   e.d_err, e.d_gpos, e.d_spk, e.d_gmeta, e.d_pf, e.d_plj, e.d_gacc, e.d_tab,
                   e.d_ljt, e.d_kres, e.d_kvec, e.d_ksc, e.d_mesh, e.d_cmesh, e.d_green, e.d_twid,
                   e.d_sgrid, e.d_sfrac, e.d_bin_fill,

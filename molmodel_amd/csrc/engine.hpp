@@ -74,6 +74,8 @@ struct Engine {
   int ncell_total = 0;
   int *d_gcell = nullptr, *d_cell_count = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr;
   int *d_order = nullptr;     // sorted gs -> original group
+  int4 *d_gslot = nullptr;    // per sorted slot: group, first site, site count, exclusion
+                              // mask (written with each neighbour-list build)
   double *d_last = nullptr;   // 3*ngroup first-site positions at the last build
   int4 *d_sten = nullptr;
   int nbcap = 0;              // neighbor slots per group

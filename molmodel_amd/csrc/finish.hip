@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
     int N, int want_virial, double bond, double angle_rad, double tol,
     double *__restrict__ part, int *__restrict__ done, int maxblk, double *__restrict__ acc,
     int *__restrict__ rebuild, int *__restrict__ err, double4 *__restrict__ gpos,
-    float4 *__restrict__ spk, const int *__restrict__ ginfo) {
+    float4 *__restrict__ spk, const int *__restrict__ ginfo, const int4 *__restrict__ gslot) {
   constexpr int NPART = FinSlots<VIR>::N, TOP = FinSlots<VIR>::TOP;
   __shared__ double red[FIN_TB / 64][NPART];
   __shared__ int islast;
@@ -57,8 +57,9 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
   for (int c = 0; c < NPART; c++) vals[c] = 0;
   if (active) {
     const int gs = P.group0 + i;
-    const int g = order[gs];
-    const int s0 = gfirst[g], ns = gns[g];
+    const int4 sl = gslot[gs];   // group, first site, site count, exclusion mask
+    const int g = sl.x;
+    const int s0 = sl.y, ns = sl.z;
     constexpr int K = 3;
     double F[K][3], ph[K], qq[K], x[K][3];
 #pragma unroll
@@ -78,7 +79,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
       }
     }
     // excluded pairs inside the group (rspace.c:492-521)
-    const int ex = gexcl[g];
+    const int ex = sl.w;
     if (ex) {
       const double L = P.L, invL = P.invL, eta = P.eta;
 #pragma unroll
@@ -297,7 +298,7 @@ int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out) {
 #define FIN_ARGS e.d_sp, e.d_order, e.d_gfirst, e.d_gns, e.d_gexcl, e.d_q, e.d_mass, e.d_pf, \
     e.d_gacc, e.d_kres, e.d_r, e.d_v, e.d_f, phi_out, e.d_last, e.nsite, want_virial, e.bond,  \
     e.angle * (M_PI / 180.0), e.tol, e.d_fpart, e.d_done, e.nblk_part, e.d_acc, e.d_rebuild,  \
-    e.d_err, e.d_gpos, e.d_spk, e.d_ginfo
+    e.d_err, e.d_gpos, e.d_spk, e.d_ginfo, e.d_gslot
 #define FIN_CASE(M)                                                                            \
   if (want_virial)                                                                             \
     hipLaunchKernelGGL((k_finish<M, true, false>), grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS);  \

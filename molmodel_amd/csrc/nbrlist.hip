@@ -173,7 +173,8 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
     int *__restrict__ nbr, int *__restrict__ nnb, int *__restrict__ err,
     const double *__restrict__ r, int N, const double *__restrict__ q,
     const int *__restrict__ gfirst, const int *__restrict__ gns, const int *__restrict__ ginfo,
-    const double4 *__restrict__ gpos, float4 *__restrict__ spk, int2 *__restrict__ gmeta) {
+    const double4 *__restrict__ gpos, float4 *__restrict__ spk, int2 *__restrict__ gmeta,
+    const int *__restrict__ gexcl, int4 *__restrict__ gslot) {
 #pragma clang fp contract(off)
   __shared__ int c_start[4][NL_SEG], c_off[4][NL_SEG + 1], c_cell[4][NL_SEG];
   const int s = blockIdx.y;
@@ -194,11 +195,13 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
   for (int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); wave < P.ngroup;
        wave += nwave) {
     const int gs = P.group0 + wave;
+    const int g = order[gs];
+    // the finish kernel's per-slot indices in the new order
+    if (lane == 0) gslot[gs] = make_int4(g, gfirst[g], gns[g], gexcl[g]);
     if (bad) {   // the error is reported; the rows stay empty
       if (lane == 0) nnb[gs] = 0;
       continue;
     }
-    const int g = order[gs];
     const int n = P.ncell;
     const int A = gcell[g] - P.cell0;
     const int ax = A / (n * n), ay = (A / n) % n, az = A % n;
@@ -390,7 +393,8 @@ int launch_nl_build(Engine &e, int force, int from_partials) {
   hipLaunchKernelGGL(k_nl_pairs, dim3((e.maxg + 3) / 4, e.nsys), dim3(256), 0,
                      e.stream, e.d_sp, e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start,
                      e.d_sten, e.nbcap, e.d_nbr, e.d_nnb, e.d_err, e.d_r, e.nsite, e.d_q,
-                     e.d_gfirst, e.d_gns, e.d_ginfo, e.d_gpos, e.d_spk, e.d_gmeta);
+                     e.d_gfirst, e.d_gns, e.d_ginfo, e.d_gpos, e.d_spk, e.d_gmeta, e.d_gexcl,
+                     e.d_gslot);
   MM_HIP(hipGetLastError());
   return 0;
 }
