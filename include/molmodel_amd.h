@@ -37,8 +37,12 @@ extern "C" {
 #define MM_ERR_HIP -2        /* HIP runtime failure */
 #define MM_ERR_DEVICE -3     /* device-side error flag raised (see mm_errors) */
 
-/* device error flags reported by mm_errors() */
+/* device error flags reported by mm_errors().  An overflow (neighbour row,
+ * P3M column bin) is not an error for the caller: the evaluation, MD block
+ * or exchange interval in which it happened runs again from its start with
+ * larger buffers. */
 #define MM_FLAG_NL_OVERFLOW 1
+#define MM_FLAG_BIN_OVERFLOW 16
 #define MM_FLAG_CONSTRAINT 4
 #define MM_FLAG_NAN 8          /* non-finite energy, or a non-finite / out-of-range position */
 #define MM_FLAG_CELL 32        /* positions collapsed: a neighbour-list cell held > 512 groups */
@@ -111,6 +115,10 @@ int mm_get_potential(mm_engine *e, double *phi);          /* nsite_total */
 int mm_get_energies(mm_engine *e, double *en);            /* MM_NE per system */
 int mm_get_virial(mm_engine *e, double *w);               /* 9 per system, row-major */
 int mm_errors(mm_engine *e, int *flags);                  /* reads and clears */
+/* raises MM_FLAG_NL_OVERFLOW and/or MM_FLAG_BIN_OVERFLOW on the device as a
+ * kernel would, so that callers can exercise the grow-and-run-again path
+ * (no reference counterpart; other flags are refused) */
+int mm_raise_flags(mm_engine *e, int flags);
 
 /* RSpaceEwald's outputs alone (rspace.c:382-558): phi, evec, lj_force (AoS)
  * and lj_energy per system, including the excluded-pair correction loop;

@@ -176,6 +176,20 @@ int mm_errors(mm_engine *m, int *flags) {
   return MM_OK;
 }
 
+int mm_raise_flags(mm_engine *m, int flags) {
+  CHECK_ENGINE(m);
+  if (flags & ~(MM_FLAG_NL_OVERFLOW | MM_FLAG_BIN_OVERFLOW)) {
+    set_last_error("mm_raise_flags: only the overflow flags can be raised");
+    return MM_ERR_ARG;
+  }
+  int f = 0;
+  TRY(read_err(m->e, &f));
+  f |= flags;
+  MM_HIP(hipMemcpyAsync(m->e.d_err, &f, sizeof(int), hipMemcpyHostToDevice, m->e.stream));
+  MM_HIP(hipStreamSynchronize(m->e.stream));
+  return MM_OK;
+}
+
 // grow the neighbour rows after an overflow and force a rebuild
 static int grow_nbr(Engine &e) {
   MM_HIP(hipStreamSynchronize(e.stream));
@@ -184,8 +198,60 @@ static int grow_nbr(Engine &e) {
   MM_HIP(hipMalloc((void **)&e.d_nbr, sizeof(int) * (size_t)e.ngroup * e.nbcap));
   e.first_eval = 1;
   // a captured step holds the old row buffer: capture again on next use
-  if (e.gexec) { hipGraphExecDestroy(e.gexec); e.gexec = nullptr; }
-  if (e.graph) { hipGraphDestroy(e.graph); e.graph = nullptr; }
+  engine_drop_graph(e);
+  return 0;
+}
+
+// Grow what the flags say overflowed (neighbour rows, P3M column bins);
+// returns 1 if the evaluation should run again, 0 if nothing overflowed,
+// or a negative status.
+static int grow_overflowed(Engine &e, int f) {
+  if (!(f & (ERR_NL_OVERFLOW | ERR_BIN_OVERFLOW))) return 0;
+  if (f & ERR_NL_OVERFLOW) TRY(grow_nbr(e));
+  if (f & ERR_BIN_OVERFLOW) TRY(engine_grow_bins(e));
+  return 1;
+}
+
+// After a non-finite or collapsed position every system's neighbour rows
+// may have been left empty (the pair search reads the one error flag all
+// systems share), while the skin test's reference positions were updated:
+// the next evaluation rebuilds every list from scratch.
+static void after_position_error(Engine &e, int f) {
+  if (f & (ERR_NAN | ERR_CELL)) e.first_eval = 1;
+}
+
+// r, v (and the exchange permutation and statistics) saved before a block
+// of steps, so that a block in which a neighbour row or a P3M bin
+// overflowed can run again with larger buffers
+static int save_state(Engine &e, int remd) {
+  const size_t n3 = 3 * (size_t)e.nsite;
+  if (!e.d_snap) MM_HIP(hipMalloc((void **)&e.d_snap, 2 * n3 * sizeof(double)));
+  MM_HIP(hipMemcpyAsync(e.d_snap, e.d_r, n3 * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
+  MM_HIP(hipMemcpyAsync(e.d_snap + n3, e.d_v, n3 * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
+  if (remd) {
+    if (!e.d_snap_perm) MM_HIP(hipMalloc((void **)&e.d_snap_perm, 3 * sizeof(int) * e.nsys));
+    if (!e.d_snap_stats) MM_HIP(hipMalloc((void **)&e.d_snap_stats, 2 * sizeof(long long) * e.nsys));
+    MM_HIP(hipMemcpyAsync(e.d_snap_perm, e.d_perm, 3 * sizeof(int) * e.nsys,
+                          hipMemcpyDeviceToDevice, e.stream));
+    MM_HIP(hipMemcpyAsync(e.d_snap_stats, e.d_remd_stats, 2 * sizeof(long long) * e.nsys,
+                          hipMemcpyDeviceToDevice, e.stream));
+  }
+  return 0;
+}
+
+static int restore_state(Engine &e, int remd) {
+  const size_t n3 = 3 * (size_t)e.nsite;
+  MM_HIP(hipMemcpyAsync(e.d_r, e.d_snap, n3 * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
+  MM_HIP(hipMemcpyAsync(e.d_v, e.d_snap + n3, n3 * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
+  if (remd) {
+    MM_HIP(hipMemcpyAsync(e.d_perm, e.d_snap_perm, 3 * sizeof(int) * e.nsys,
+                          hipMemcpyDeviceToDevice, e.stream));
+    MM_HIP(hipMemcpyAsync(e.d_remd_stats, e.d_snap_stats, 2 * sizeof(long long) * e.nsys,
+                          hipMemcpyDeviceToDevice, e.stream));
+  }
+  e.need_pack = 1;
+  e.first_eval = 1;
+  e.md_phase = 0;
   return 0;
 }
 
@@ -234,27 +300,29 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial,
 
 static int compute_checked(mm_engine *m, int field_mode, int want_virial) {
   Engine &e = m->e;
-  for (int attempt = 0; attempt < 4; attempt++) {
+  for (int attempt = 0; attempt < 16; attempt++) {
     TRY(enqueue_forces(m, field_mode, want_virial));
     if (!field_mode) TRY(launch_finish_ex(e, FIN_FORCES, want_virial, e.d_phi));
     int f = 0;
     TRY(read_err(e, &f));
-    if (f & (ERR_NAN | ERR_CONSTRAINT | ERR_CELL | ERR_BIN_OVERFLOW)) {   // reported once, before any retry
+    if (f & (ERR_NAN | ERR_CONSTRAINT | ERR_CELL)) {   // reported once, before any retry
       TRY(clear_err(e));
+      after_position_error(e, f);
       set_last_error(f & ERR_NAN ? "non-finite (or out-of-range) position or energy during force evaluation"
                      : f & ERR_CELL ? "positions collapsed: a neighbour-list cell is overfull"
-                     : f & ERR_BIN_OVERFLOW ? "a P3M mesh column bin overflowed (sites far denser than the box average)"
                                     : "device error flag raised during force evaluation");
       return MM_ERR_DEVICE;
     }
-    if (f & ERR_NL_OVERFLOW) {
-      TRY(clear_err(e));
-      TRY(grow_nbr(e));
+    if (f) TRY(clear_err(e));
+    const int grown = grow_overflowed(e, f);
+    if (grown < 0) return grown;
+    if (grown) {
+      m->pos_dirty = 1;   // pack again (a retry repeats the whole evaluation)
       continue;
     }
     return 0;
   }
-  set_last_error("neighbour list overflow persisted");
+  set_last_error("neighbour list or P3M bin overflow persisted");
   return MM_ERR_DEVICE;
 }
 
@@ -496,8 +564,7 @@ int mm_scale_system(mm_engine *m, int sys, double f) {
   MM_HIP(hipStreamSynchronize(e.stream));
   TRY(engine_scale_system(e, sys, f));
   // the captured step holds the old stencil buffer
-  if (e.gexec) { hipGraphExecDestroy(e.gexec); e.gexec = nullptr; }
-  if (e.graph) { hipGraphDestroy(e.graph); e.graph = nullptr; }
+  engine_drop_graph(e);
   m->pos_dirty = 1;
   e.md_phase = 0;
   return MM_OK;
@@ -514,8 +581,7 @@ int mm_md_setup(mm_engine *m, double dt, double bond, double angle_deg, double t
   e.tol = tol;
   for (auto &P : e.hsp) P.dt = dt;
   MM_HIP(hipMemcpy(e.d_sp, e.hsp.data(), sizeof(SysParams) * e.nsys, hipMemcpyHostToDevice));
-  if (e.gexec) { hipGraphExecDestroy(e.gexec); e.gexec = nullptr; }
-  if (e.graph) { hipGraphDestroy(e.graph); e.graph = nullptr; }
+  engine_drop_graph(e);
   TRY(launch_constrain_initial(e));   // mdyn.cpp:120-121
   m->pos_dirty = 1;
   TRY(compute_checked(m, 0, 0));      // mdyn.cpp:122-124
@@ -563,19 +629,41 @@ static int md_run_async(mm_engine *m, int nsteps, int use_graph) {
   return 0;
 }
 
+static int md_report(Engine &e, int f) {
+  if (!f) return MM_OK;
+  TRY(clear_err(e));   // reported once
+  after_position_error(e, f);
+  set_last_error(f & ERR_NAN ? "non-finite (or out-of-range) position or energy during MD"
+                 : f & ERR_CELL ? "positions collapsed during MD: a neighbour-list cell is overfull"
+                 : f & ERR_BIN_OVERFLOW ? "a P3M mesh column bin overflowed during MD"
+                 : f & ERR_CONSTRAINT ? "constraint iteration did not converge"
+                                      : "neighbour list overflow during MD");
+  return MM_ERR_DEVICE;
+}
+
 static int md_check_errors(Engine &e) {
   int f = 0;
   TRY(read_err(e, &f));
-  if (f) {
-    TRY(clear_err(e));   // reported once
-    set_last_error(f & ERR_NAN ? "non-finite (or out-of-range) position or energy during MD"
-                   : f & ERR_CELL ? "positions collapsed during MD: a neighbour-list cell is overfull"
-                   : f & ERR_BIN_OVERFLOW ? "a P3M mesh column bin overflowed during MD"
-                   : f & ERR_CONSTRAINT ? "constraint iteration did not converge"
-                                        : "neighbour list overflow during MD");
-    return MM_ERR_DEVICE;
+  return md_report(e, f);
+}
+
+// An overflow alone (neighbour row or P3M bin, with no position error)
+// makes the block run again from its saved start with larger buffers.
+static bool replayable(int f) {
+  return (f & (ERR_NL_OVERFLOW | ERR_BIN_OVERFLOW)) && !(f & (ERR_NAN | ERR_CELL | ERR_CONSTRAINT));
+}
+
+// forces of the current positions before a block starts: positions set by
+// the caller, a box scaling or a recovered error leave the last
+// evaluation's pair and reciprocal-space results stale, and the first half
+// kick (and any captured step) must not use them
+static int forces_current(mm_engine *m) {
+  Engine &e = m->e;
+  if (e.first_eval || m->pos_dirty) {
+    TRY(compute_checked(m, 0, 0));
+    e.md_phase = 0;
   }
-  return MM_OK;
+  return 0;
 }
 
 int mm_md_run(mm_engine *m, int nsteps, int use_graph) {
@@ -583,8 +671,23 @@ int mm_md_run(mm_engine *m, int nsteps, int use_graph) {
   Engine &e = m->e;
   if (!e.md_ready) { set_last_error("mm_md_setup first"); return MM_ERR_ARG; }
   if (nsteps <= 0) return MM_OK;
-  TRY(md_run_async(m, nsteps, use_graph));
-  return md_check_errors(e);
+  TRY(forces_current(m));
+  for (int attempt = 0;; attempt++) {
+    TRY(save_state(e, 0));
+    TRY(md_run_async(m, nsteps, use_graph));
+    int f = 0;
+    TRY(read_err(e, &f));
+    if (replayable(f) && attempt < 8) {
+      TRY(clear_err(e));
+      const int g = grow_overflowed(e, f);
+      if (g < 0) return g;
+      TRY(restore_state(e, 0));
+      m->pos_dirty = 1;
+      TRY(forces_current(m));
+      continue;
+    }
+    return md_report(e, f);
+  }
 }
 
 int mm_remd_setup(mm_engine *m, int nladder, const int *ladder, const double *kT,
@@ -642,22 +745,41 @@ int mm_remd_run(mm_engine *m, int nintervals, int steps, int reinit_velocities, 
   Engine &e = m->e;
   if (!e.d_ladder) { set_last_error("mm_remd_setup first"); return MM_ERR_ARG; }
   if (nintervals < 0 || steps < 0) { set_last_error("bad interval arguments"); return MM_ERR_ARG; }
+  TRY(forces_current(m));
   for (int it = 0; it < nintervals; it++) {
-    if (steps > 0) TRY(md_run_async(m, steps, use_graph));
-    const long long a = e.remd_attempt++;
-    // end of the interval's last step: the exchange attempt (mdyn.cpp:185-186),
-    // then the velocity redraw (mdyn.cpp:163-167; the reference redraws
-    // before exchanging, which leaves swapped rigid molecules with velocities
-    // constrained for the configuration they left)
-    TRY(launch_remd_exchange(e, a));
-    if (reinit_velocities) TRY(launch_remd_velocities(e, a));
-    e.need_pack = 1;   // positions moved between systems
-    // forces and energies of the configurations now held; swapped systems
-    // rebuild their neighbour lists first
-    TRY(enqueue_forces(m, 0, 0));
-    TRY(launch_finish_ex(e, FIN_FORCES, 0, e.d_phi));
+    for (int attempt = 0;; attempt++) {
+      TRY(save_state(e, 1));
+      if (steps > 0) TRY(md_run_async(m, steps, use_graph));
+      const long long a = e.remd_attempt;
+      // end of the interval's last step: the exchange attempt (mdyn.cpp:185-186),
+      // then the velocity redraw (mdyn.cpp:163-167; the reference redraws
+      // before exchanging, which leaves swapped rigid molecules with velocities
+      // constrained for the configuration they left)
+      TRY(launch_remd_exchange(e, a));
+      if (reinit_velocities) TRY(launch_remd_velocities(e, a));
+      e.need_pack = 1;   // positions moved between systems
+      // forces and energies of the configurations now held; swapped systems
+      // rebuild their neighbour lists first
+      TRY(enqueue_forces(m, 0, 0));
+      TRY(launch_finish_ex(e, FIN_FORCES, 0, e.d_phi));
+      int f = 0;
+      TRY(read_err(e, &f));
+      if (replayable(f) && attempt < 8) {
+        // the interval runs again from its start with the same random numbers
+        TRY(clear_err(e));
+        const int g = grow_overflowed(e, f);
+        if (g < 0) return g;
+        TRY(restore_state(e, 1));
+        m->pos_dirty = 1;
+        TRY(forces_current(m));
+        continue;
+      }
+      if (f) return md_report(e, f);
+      break;
+    }
+    e.remd_attempt++;
   }
-  return md_check_errors(e);
+  return MM_OK;
 }
 
 int mm_remd_stats(mm_engine *m, long long *tries_accepts, int *config) {

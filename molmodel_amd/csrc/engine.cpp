@@ -691,6 +691,37 @@ int engine_scale_system(Engine &e, int s, double f) {
   return 0;
 }
 
+void engine_drop_graph(Engine &e) {
+  if (e.gexec) { hipGraphExecDestroy(e.gexec); e.gexec = nullptr; }
+  if (e.graph) { hipGraphDestroy(e.graph); e.graph = nullptr; }
+}
+
+int engine_grow_bins(Engine &e) {
+  MM_HIP(hipStreamSynchronize(e.stream));
+  if (e.stream2) MM_HIP(hipStreamSynchronize(e.stream2));
+  int need = 0, zero = 0;
+  MM_HIP(hipMemcpy(&need, e.d_bin_fill + e.nbin_total, sizeof(int), hipMemcpyDeviceToHost));
+  MM_HIP(hipMemcpy(e.d_bin_fill + e.nbin_total, &zero, sizeof(int), hipMemcpyHostToDevice));
+  // a bin never holds more sites than its system has
+  int most = 32;
+  for (auto &P : e.hsp)
+    if (P.kmode == 1) most = std::max(most, P.nsite);
+  most = (most + 31) / 32 * 32;
+  long want = std::max(2L * e.bcap, (long)need + need / 4 + 32);
+  want = std::min<long>((want + 31) / 32 * 32, most);
+  MM_REQ(want > e.bcap, "P3M column bins cannot grow further");
+  const size_t nb = (size_t)std::max(e.nbin_total, 1);
+  MM_HIP(hipFree(e.d_bgrid));
+  MM_HIP(hipFree(e.d_bfrac));
+  e.d_bgrid = nullptr;
+  e.d_bfrac = nullptr;
+  e.bcap = (int)want;
+  MM_TRY(dalloc(&e.d_bgrid, nb * e.bcap));
+  MM_TRY(dalloc(&e.d_bfrac, nb * e.bcap));
+  engine_drop_graph(e);
+  return 0;
+}
+
 void engine_destroy(Engine &e) {
   if (e.gexec) hipGraphExecDestroy(e.gexec);
   if (e.graph) hipGraphDestroy(e.graph);
@@ -702,7 +733,7 @@ void engine_destroy(Engine &e) {
                   e.d_sgrid, e.d_sfrac, e.d_bin_fill,
                   e.d_bgrid, e.d_bfrac, e.d_acc, e.d_part, e.d_fpart, e.d_done, e.d_phi, e.d_scratch,
                   e.d_nlb, e.d_ladder, e.d_kT, e.d_tail, e.d_rtmp, e.d_mom, e.d_perm,
-                  e.d_remd_stats, e.d_tile_start};
+                  e.d_remd_stats, e.d_tile_start, e.d_snap, e.d_snap_perm, e.d_snap_stats};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (e.stream) hipStreamDestroy(e.stream);

@@ -150,6 +150,12 @@ struct Engine {
   long long *d_remd_stats = nullptr;
   unsigned long long remd_seed = 0;
   long long remd_attempt = 0;
+  // state saved at the start of an MD block or exchange interval, restored
+  // when an overflow makes the block run again (r, v: 6 N; exchange
+  // permutation and statistics)
+  double *d_snap = nullptr;
+  int *d_snap_perm = nullptr;
+  long long *d_snap_stats = nullptr;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -165,6 +171,11 @@ void cheb_fit(double a, double b, double (*f)(double), double *c4);
 void engine_destroy(Engine &e);
 void lj_tail_factors(const SystemInput &s, double *ca, double *cb);
 int engine_scale_system(Engine &e, int s, double f);
+// drop the captured step graph (it holds buffer addresses that changed)
+void engine_drop_graph(Engine &e);
+// after ERR_BIN_OVERFLOW: grow the P3M column bins to the capacity the
+// binning kernel recorded (at least twice the old one) and clear the record
+int engine_grow_bins(Engine &e);
 
 // launchers (each returns 0 or a negative status)
 // from_partials: the rebuild test of each system also fires when the

@@ -72,6 +72,9 @@ __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >=
 // in LDS and claims each bin's slots with one global atomic; the order
 // inside a bin varies from run to run, and the spreading's 64-bit
 // fixed-point sums do not depend on it.  The gather clears the counters.
+// A site beyond a bin's capacity raises ERR_BIN_OVERFLOW and records in
+// *need the slot count that bin would have needed, so the host can grow the
+// bins and evaluate again.
 constexpr int MAXBINS = 4096;
 __global__ void __launch_bounds__(256) k_p3m_bin(const SysParams *__restrict__ sp,
                                                  const double *__restrict__ r,
@@ -81,7 +84,7 @@ __global__ void __launch_bounds__(256) k_p3m_bin(const SysParams *__restrict__ s
                                                  int *__restrict__ fill, int bcap,
                                                  int4 *__restrict__ bgrid,
                                                  float4 *__restrict__ bfrac,
-                                                 int *__restrict__ err) {
+                                                 int *__restrict__ err, int *__restrict__ need) {
   __shared__ int cnt[MAXBINS];
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
@@ -128,6 +131,7 @@ __global__ void __launch_bounds__(256) k_p3m_bin(const SysParams *__restrict__ s
     bfrac[(long)b * bcap + slot] = f;
   } else {
     atomicOr(err, ERR_BIN_OVERFLOW);   // reported; the site is left out of the mesh
+    atomicMax(need, slot + 1);
   }
 }
 
@@ -559,7 +563,8 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies) {
   const int tb = 256;
   dim3 gs((maxn + tb - 1) / tb, e.nsys);
   hipLaunchKernelGGL(k_p3m_bin, gs, dim3(tb), 0, st, e.d_sp, e.d_r, e.d_q, e.nsite,
-                     e.d_sgrid, e.d_sfrac, e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_err);
+                     e.d_sgrid, e.d_sfrac, e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_err,
+                     e.d_bin_fill + e.nbin_total);
   bool used[MAXORDER + 1] = {false};
   for (auto &p : e.hsp)
     if (p.kmode == 1) used[p.order] = true;

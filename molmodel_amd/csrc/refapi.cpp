@@ -590,6 +590,7 @@ extern "C" p3m_t p3m_new(int nsite, const double ptv[9], double grid_spacing, in
     die_msg("p3m_new", "mesh dimensions give no valid spreading column blocks");
   P.nbx = P.nx / P.bx;
   P.nby = P.ny / P.by;
+  e.nbin_total = P.nbx * P.nby;   // d_bin_fill[nbin_total] records an overfull bin's need
   e.nblk_part = std::max(1, (P.ny * nzh + P3M_FFT_LINES - 1) / P3M_FFT_LINES);
   const size_t M = (size_t)P.nx * P.ny * P.nz, Mc = (size_t)P.nx * P.ny * nzh;
   HIPCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
@@ -665,13 +666,16 @@ extern "C" void p3m_field(p3m_t p, const double *r, const double *q, double *phi
   to_soa(r, N, soa);
   HIPCHK(hipMemcpy(e.d_r, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(e.d_q, q, sizeof(double) * N, hipMemcpyHostToDevice));
-  if (launch_p3m(e, 0)) die_msg("p3m_field", last_error());
-  HIPCHK(hipStreamSynchronize(e.stream));
-  int flags = 0;
-  HIPCHK(hipMemcpy(&flags, e.d_err, sizeof(int), hipMemcpyDeviceToHost));
-  if (flags & ERR_BIN_OVERFLOW) {
+  // sites far denser than the box average (a slab, a cluster) overflow the
+  // column bins: grow them to what the binning recorded and evaluate again
+  for (int attempt = 0;; attempt++) {
+    if (launch_p3m(e, 0)) die_msg("p3m_field", last_error());
+    HIPCHK(hipStreamSynchronize(e.stream));
+    int flags = 0;
+    HIPCHK(hipMemcpy(&flags, e.d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (!(flags & ERR_BIN_OVERFLOW)) break;
     HIPCHK(hipMemset(e.d_err, 0, sizeof(int)));
-    die_msg("p3m_field", "a mesh column bin overflowed (sites far denser than the box average)");
+    if (attempt >= 8 || engine_grow_bins(e)) die_msg("p3m_field", "mesh column bins cannot grow further");
   }
   std::vector<double> out(4 * (size_t)N);
   HIPCHK(hipMemcpy(out.data(), e.d_kres, sizeof(double) * out.size(), hipMemcpyDeviceToHost));

@@ -148,7 +148,10 @@ extern "C" int mm_tune_kspace(double boxl, int nsite, double rc, double tol, int
     }
   }
   const std::vector<int> sizes = smooth_sizes();
-  for (int order = 3; order <= 7; order++) {
+  // orders the reference can run: its assignment weights stop at order 6
+  // (order 7 falls through to assert(0), pppm.c:147-171), so a choice of 7
+  // would have no reference result to be compared with
+  for (int order = 3; order <= 6; order++) {
     // smallest size meeting the target (chi falls with the mesh size):
     // galloping search upwards from the smallest size, then bisection
     const int last = (int)sizes.size() - 1;

@@ -53,6 +53,7 @@ def lib():
             getattr(L, name).argtypes = [_P, _P]
         L.mm_compute.argtypes = [_P, _I]
         L.mm_errors.argtypes = [_P, _P]
+        L.mm_raise_flags.argtypes = [_P, _I]
         L.mm_rspace_field.argtypes = [_P, _P, _P, _P, _P]
         L.mm_kspace_field.argtypes = [_P, _P, _P]
         L.mm_nl_pairs.restype = _L
@@ -215,6 +216,10 @@ class Engine:
         f = np.zeros(1, dtype=np.int32)
         _check(lib().mm_errors(self.h, _ptr(f)))
         return int(f[0])
+
+    def raise_flags(self, flags):
+        """Raise overflow flags on the device (tests of the recovery path)."""
+        _check(lib().mm_raise_flags(self.h, int(flags)))
 
     def rspace_field(self):
         phi = np.zeros(self.nsite); ev = np.zeros((self.nsite, 3)); ljf = np.zeros((self.nsite, 3))

@@ -194,6 +194,28 @@ def water_box(nside: int, boxl: float, seed: int = 2024, temperature: float = 30
                   meta=dict(seed=seed, nside=nside, temperature=temperature))
 
 
+def subset(s: System, groups) -> System:
+    """The system made of the given groups of s (in that order), in the same
+    box; exclusions inside the kept groups are kept."""
+    groups = np.asarray(groups, dtype=np.int64)
+    sites, new_group, where = [], [], {}
+    for g in groups:
+        a, b = int(s.group[g, 0]), int(s.group[g, 1])
+        new_group.append((len(sites), len(sites) + b - a))
+        for i in range(a, b):
+            where[i] = len(sites)
+            sites.append(i)
+    sites = np.asarray(sites, dtype=np.int64)
+    excl = [(where[i], where[j]) for i, j in s.excl.tolist() if i in where and j in where]
+    nw = int(sum(1 for g in groups if g < s.nwater))
+    return System(boxl=s.boxl, r=s.r[sites].copy(), v=s.v[sites].copy(), q=s.q[sites].copy(),
+                  sig=s.sig[sites].copy(), eps=s.eps[sites].copy(),
+                  site_mask=s.site_mask[sites].copy(), mass=s.mass[sites].copy(),
+                  group=np.asarray(new_group, dtype=np.int32).reshape(-1, 2),
+                  excl=np.asarray(excl, dtype=np.int32).reshape(-1, 2), nwater=nw,
+                  nion=len(groups) - nw, meta=dict(s.meta, subset=len(groups)))
+
+
 def thermal_velocities(mass, group, r, nwater, temperature, seed):
     """Maxwell-Boltzmann velocities; each water's velocities are projected
     onto its rigid-body (constraint-preserving) subspace; total momentum 0."""

@@ -169,8 +169,9 @@ def evaluate(system, p, r=None, virial=False):
     return (f, en, vir) if virial else (f, en)
 
 
-def rspace_parts(system, p, r=None):
-    """phi, evec, lj_force, lj_energy of RSpaceEwald alone (oracle)."""
+def rspace_parts(system, p, r=None, virial=False):
+    """phi, evec, lj_force, lj_energy of RSpaceEwald alone (oracle); with
+    virial, also its 3x3 virial sum (row-major r_a F_b)."""
     lib = oracle()
     r = np.ascontiguousarray(system.r if r is None else r, dtype=np.float64)
     gr = first_sites(system, r)
@@ -181,13 +182,16 @@ def rspace_parts(system, p, r=None):
     lib.orc_cheby_rspace(cheb, p["rc"], p["eta"])
     n = system.nsite
     phi = np.zeros(n); ev = np.zeros((n, 3)); ljf = np.zeros((n, 3)); lje = np.zeros(1)
+    vir = np.zeros(9) if virial else None
     lib.orc_rspace_ewald(n, system.ngroup, _ptr(np.ascontiguousarray(system.group)), _ptr(r),
                          system.boxl, len(nl), _ptr(nl), len(system.excl),
                          _ptr(np.ascontiguousarray(system.excl)),
                          (p["rc"] - p["smoothing_width"]) ** 2, p["rc"] ** 2,
                          _ptr(system.site_mask), _ptr(system.q), p["eta"], cheb, None,
                          _ptr(system.sig), _ptr(system.eps), 0, 0, None, 1.0, 1.0,
-                         _ptr(phi), _ptr(ev), _ptr(lje), _ptr(ljf), None)
+                         _ptr(phi), _ptr(ev), _ptr(lje), _ptr(ljf), _ptr(vir))
+    if virial:
+        return phi, ev, ljf, lje[0], vir.reshape(3, 3)
     return phi, ev, ljf, lje[0]
 
 

@@ -8,8 +8,17 @@ Bars:
   * constraints hold to the iteration tolerance: |0.5 (r_OH^2 - 1)| <= 1e-8,
     |theta - theta0| <= 1e-8;
   * a captured hipGraph replay gives bit-identical states to stream launches;
-  * 1000 steps conserve the total energy to 2% of the kinetic energy.
+  * 1000 steps of the Ewald box: the total energy's excursion stays within
+    twice the FP64 oracle's own over the same run, and the final total energy
+    is the oracle's to 0.5% of the kinetic energy;
+  * 500 steps of the 12 288-atom P3M box: per-step potential energy within
+    1e-5 of the oracle's run for 100 steps and 1e-4 for all 500, total-energy
+    drift equal to the oracle's within 2e-4 of the kinetic energy, and the
+    step-500 state evaluated by both within the single-snapshot bars.
 """
+import hashlib
+import os
+
 import numpy as np
 import pytest
 
@@ -18,6 +27,12 @@ from molmodel_amd import waterbox as W
 from molmodel_amd.engine import Engine
 
 pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
 
 
 def setup(name, seed=2024):
@@ -72,19 +87,77 @@ def test_graph_replay_bit_identical(gpu_lib):
     assert np.array_equal(e1.positions(), e2.positions())
 
 
+def oracle_run(fname, s):
+    g = np.load(os.path.join(GOLD, fname), allow_pickle=False)
+    assert str(g["r_digest"][0]) == digest(s.r) and str(g["v_digest"][0]) == digest(s.v)
+    return g["energies"]
+
+
 def test_energy_conservation_1000_steps(gpu_lib):
+    """1000 steps of the 648-atom Ewald box: the total energy's excursion from
+    its step-1 value stays within twice the FP64 oracle's own over the same
+    run (tests/golden/orc_md_water648_1000.npz), and the step-1000 total energy
+    differs from the oracle's by less than its kinetic energy's 0.5%."""
     s, p, e = setup("water648_ewald")
-    e0 = e.energies()[0]
-    e.md_run(1, graph=False)
-    first = e.energies()[0]
-    etot0 = first[3] + first[4]
-    e.md_run(999, graph=True)
+    ref = oracle_run("orc_md_water648_1000.npz", s)
+    etot = []
+    for chunk in [1] + [111] * 9:
+        e.md_run(chunk, graph=True)
+        en = e.energies()[0]
+        etot.append(en[3] + en[4])
+    steps = np.cumsum([1] + [111] * 9)
+    ref_tot = ref[:, 0] + ref[:, 1]
+    gpu_exc = np.abs(np.array(etot) - etot[0]).max()
+    orc_exc = np.abs(ref_tot[steps - 1] - ref_tot[0]).max()
     last = e.energies()[0]
-    etot1 = last[3] + last[4]
-    print(f"E0 {etot0:.5f} E1000 {etot1:.5f} KE {last[4]:.3f} drift {etot1 - etot0:.5f}")
+    print(f"[md] 1000 steps: GPU excursion {gpu_exc:.5f}, oracle {orc_exc:.5f}, KE {last[4]:.3f}, "
+          f"E1000 {etot[-1]:.5f} vs oracle {ref_tot[-1]:.5f}")
     assert e.errors() == 0
-    assert abs(etot1 - etot0) <= 0.02 * last[4]
+    assert gpu_exc <= 2 * orc_exc + 1e-3
+    assert abs(etot[-1] - ref_tot[-1]) <= 0.005 * last[4]
     assert max(water_geometry_errors(e.positions())) <= 1e-8
+
+
+def test_p3m_500_steps_against_oracle_run(gpu_lib):
+    """The 12 288-atom P3M box (48^3 mesh, order 5) for 500 steps: the
+    per-step potential energy follows the FP64 oracle's own 500-step run
+    (tests/golden/orc_md_water12k_500.npz), the total-energy drift agrees
+    with the oracle's, and the step-500 state re-evaluated by the oracle
+    gives the GPU's forces, energies and virial within 1e-5."""
+    s, p, e = setup("water12k_p3m")
+    ref = oracle_run("orc_md_water12k_500.npz", s)
+    print("[md] water12k p3m 500 steps", flush=True)
+    pot, kin = [], []
+    for k in range(500):
+        e.md_run(1, graph=False)
+        en = e.energies()[0]
+        pot.append(en[3])
+        kin.append(en[4])
+    pot, kin = np.array(pot), np.array(kin)
+    du = np.abs(pot - ref[:, 0]) / np.abs(ref[:, 0])
+    dk = np.abs(kin - ref[:, 1]) / ref[:, 1]
+    drift_gpu = (pot[-1] + kin[-1]) - (pot[0] + kin[0])
+    drift_orc = (ref[-1, 0] + ref[-1, 1]) - (ref[0, 0] + ref[0, 1])
+    print(f"[md] dU/|U| max {du.max():.2e} (first 100 {du[:100].max():.2e}), dKE/KE max {dk.max():.2e}, "
+          f"drift GPU {drift_gpu:.4f} oracle {drift_orc:.4f} KE {kin[-1]:.2f}", flush=True)
+    assert du[:100].max() <= 1e-5
+    assert du.max() <= 1e-4
+    assert dk.max() <= 1e-3
+    assert abs(drift_gpu - drift_orc) <= 2e-4 * kin[-1]
+    assert max(water_geometry_errors(e.positions())) <= 1e-8
+    # the step-500 state, evaluated by both
+    r = e.positions()
+    e.compute(virial=True)
+    of, oen, ow = O.evaluate(s, p, r=r, virial=True)
+    f, en, w = e.forces(), e.energies()[0], e.virial()[0]
+    ferr = np.linalg.norm(f - of) / np.linalg.norm(of)
+    werr = np.linalg.norm(w - ow.reshape(3, 3)) / np.linalg.norm(ow)
+    print(f"[md] step 500: force rel {ferr:.2e}, dU {abs(en[3] - oen[3]) / abs(oen[3]):.2e}, "
+          f"virial rel {werr:.2e}")
+    assert ferr <= 1e-5
+    assert abs(en[0] - oen[0]) <= 1e-5 * abs(oen[3])
+    assert abs(en[1] - oen[1]) <= 1e-5 * abs(oen[3])
+    assert werr <= 1e-5
 
 
 def test_p3m_md_runs(gpu_lib):

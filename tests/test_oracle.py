@@ -196,8 +196,11 @@ def test_phi_at_gridpoints_is_linear_in_charges():
     assert np.allclose(b, 2.5 * a, rtol=1e-13, atol=1e-13)
 
 
-@pytest.mark.skipif(O.ref() is None, reason="reference sources not built in this environment")
 def test_rspace_matches_live_reference_other_seed():
+    # the compiled reference is looked up only when this test runs (never at
+    # collection), and only where its sources are present
+    if not os.path.isdir("/root/reference/src") or O.ref() is None:
+        pytest.skip("reference sources not built in this environment")
     s, p = W.config("water648_ewald", seed=77)
     a = O.rspace_parts(s, p)
     b = O.ref_rspace_parts(s, p)

@@ -55,3 +55,12 @@ def test_chi_tracks_measured_p3m_error():
         chis.append(chi)
     assert max(chis) / min(chis) > 300
     assert 1.0 < min(ratios) and max(ratios) < 10.0, ratios
+
+
+@pytest.mark.parametrize("boxl,nsite,rc", SIZES)
+def test_orders_the_reference_runs(boxl, nsite, rc):
+    """The reference's assignment weights end at order 6 (order 7 falls
+    through to assert(0), pppm.c:147-171): the tuner never picks 7."""
+    for tol in (1e-3, 1e-4, 1e-5):
+        c = tune_kspace(boxl, nsite, rc, tol, allow_ewald=False)
+        assert c["kmode"] == 1 and 3 <= c["order"] <= 6, c
