@@ -202,19 +202,48 @@ int mm_tune_kspace(double boxl, int nsite, double rc, double tol, int allow_ewal
  * parameter interpolation, eleclj.cpp:273-319).  Site types are the
  * distinct (q, sigma, epsilon, mask) of the system's sites, numbered in
  * order of first appearance among the sites; the per-type parameters are
- * (q [internal charge units], sigma, epsilon).  Forces are bilinear in the
- * charges and linear in each LJ type pair's a = 4 e s^12, b = 4 e s^6, so
- * the gradients come from a small basis of device force evaluations at
- * the current positions (pair terms recomputed); sigma and epsilon through
- * the mixing rule on the host.  Outputs (any may be NULL except ntype):
- * ntype; site_type[nsite]; type_params[3*ntype]; dE, dL [3*ntype] ordered
- * (dq, dsigma, depsilon) per type; loss; basis_err = relative RMS
- * difference between the forces reassembled from the basis and the
- * evaluated forces.  f_ref may be NULL (energy gradients only).  Leaves
- * the engine's parameters and forces as they were. */
+ * (q [internal charge units], sigma, epsilon).  One forward evaluation at
+ * the current positions, then the adjoint kernels: one pass over the
+ * neighbour rows that recomputes every pair term and accumulates, per site,
+ * R . dF/dq (R = 2 (F - f_ref)) and, per LJ type pair, the derivatives of E
+ * and L with respect to a = 4 e s^12 and b = 4 e s^6; the reciprocal-space
+ * adjoint (Ewald adjoint structure factors, or one P3M spread of the
+ * adjoint source with the forward's convolution and gather); sigma and
+ * epsilon through the mixing rule on the host.  Pair arithmetic FP32,
+ * accumulation FP64.  Outputs (any may be NULL except ntype): ntype;
+ * site_type[nsite]; type_params[3*ntype]; dE, dL [3*ntype] ordered
+ * (dq, dsigma, depsilon) per type; loss; ms = device time of the adjoint
+ * passes.  f_ref may be NULL (energy gradients only).  Leaves the engine's
+ * parameters and forces as they were (the next evaluation recomputes the
+ * reciprocal-space buffers the adjoint reused). */
 int mm_param_gradients(mm_engine *e, int sys, const double *f_ref, int maxtypes, int *ntype,
                        int *site_type, double *type_params, double *dE, double *dL,
-                       double *loss, double *basis_err);
+                       double *loss, double *ms);
+/* The same gradients from a basis of forward evaluations (forces are
+ * bilinear in the type charges and linear in each LJ type pair's a, b):
+ * indicator charges and unit LJ entries, reassembled on the host.  An
+ * independent check of mm_param_gradients; basis_err = relative RMS
+ * difference between the reassembled and the evaluated forces. */
+int mm_param_gradients_basis(mm_engine *e, int sys, const double *f_ref, int maxtypes,
+                             int *ntype, int *site_type, double *type_params, double *dE,
+                             double *dL, double *loss, double *basis_err);
+
+/* New per-site charges (internal units), sigma and epsilon for system sys
+ * (site masks, groups and exclusions stay): what ElecLJ::set_lambda leaves
+ * behind for new site parameters (eleclj.cpp:273-308) -- LJ types and
+ * tables, the tail correction over the "%12.6g" keys and the neutralising
+ * background follow.  At most 8 distinct (sigma, epsilon) per system. */
+int mm_set_site_params(mm_engine *e, int sys, const double *q, const double *sigma,
+                       const double *epsilon);
+/* Free-energy perturbation (Potential::types_changed(pert) + set_lambda,
+ * eleclj.cpp:267-308, msimprog.cpp:80-114): set_perturbation records the
+ * system's current parameters as state A and the given ones as state B;
+ * set_lambda(l) then sets q = (1 - l) qA + l qB for every site and sigma,
+ * epsilon likewise for sites with LJ.  Systems of a batch hold their own
+ * lambda, so FEP windows run as one launch. */
+int mm_set_perturbation(mm_engine *e, int sys, const double *qB, const double *sigmaB,
+                        const double *epsilonB);
+int mm_set_lambda(mm_engine *e, int sys, double lambda);
 
 /* interaction counts of the current neighbour list, 3 per system:
  * group pairs with first-site r^2 <= rc^2 (unordered), charged site pairs

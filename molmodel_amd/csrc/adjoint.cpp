@@ -35,6 +35,12 @@
 
 using namespace mmamd;
 
+#define TRY_ADJ(x)        \
+  do {                    \
+    int rc_ = (x);        \
+    if (rc_) return rc_;  \
+  } while (0)
+
 namespace {
 
 int put_q(Engine &e, int s, const std::vector<double> &q) {
@@ -81,11 +87,11 @@ double dot(const std::vector<double> &a, const std::vector<double> &b) {
 
 }  // namespace
 
-extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, int maxtypes,
-                                  int *ntype_out, int *site_type, double *type_params,
-                                  double *dE, double *dL, double *loss, double *basis_err) {
+extern "C" int mm_param_gradients_basis(mm_engine *m, int sys, const double *f_ref, int maxtypes,
+                                        int *ntype_out, int *site_type, double *type_params,
+                                        double *dE, double *dL, double *loss, double *basis_err) {
   if (!m || !ntype_out || sys < 0 || sys >= m->e.nsys || maxtypes <= 0) {
-    set_last_error("mm_param_gradients: bad arguments");
+    set_last_error("mm_param_gradients_basis: bad arguments");
     return MM_ERR_ARG;
   }
   Engine &e = m->e;
@@ -278,4 +284,295 @@ extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, in
   if (dL)
     for (int k = 0; k < 3 * nt; k++) dL[k] = gL[k];
   return MM_OK;
+}
+
+// ------------------------------------------------------------ adjoint path
+namespace {
+
+template <class T>
+int ensure(T **p, size_t n) {
+  if (*p) return 0;
+  MM_HIP(hipMalloc((void **)p, (n ? n : 1) * sizeof(T)));
+  MM_HIP(hipMemset(*p, 0, (n ? n : 1) * sizeof(T)));
+  return 0;
+}
+
+}  // namespace
+
+// Gradients with every pair term recomputed by the adjoint kernels
+// (adjoint.hip): one pass over the neighbour rows for the real-space and
+// excluded-pair terms, the reciprocal-space adjoint (Ewald structure factors
+// or one P3M spread + convolution + gather), reduced per site type on the
+// host.  Same site types, outputs and chain rule through the mixing rule as
+// mm_param_gradients_basis, which evaluates a basis of forward passes
+// instead.  ms: device time of the adjoint passes (may be NULL).
+extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, int maxtypes,
+                                  int *ntype_out, int *site_type, double *type_params,
+                                  double *dE, double *dL, double *loss, double *ms) {
+  if (!m || !ntype_out || sys < 0 || sys >= m->e.nsys || maxtypes <= 0) {
+    set_last_error("mm_param_gradients: bad arguments");
+    return MM_ERR_ARG;
+  }
+  Engine &e = m->e;
+  const SystemInput &S = e.in[sys];
+  const SysParams &P = e.hsp[sys];
+  const int N = S.nsite, NT = e.nsite;
+  // site types: distinct (q, sigma, epsilon, mask), in order of appearance
+  std::map<std::tuple<double, double, double, int>, int> tmap;
+  std::vector<int> ty(N);
+  for (int i = 0; i < N; i++) {
+    auto key = std::make_tuple(S.q[i], S.sig[i], S.eps[i], S.mask[i] & 3);
+    auto it = tmap.find(key);
+    if (it == tmap.end()) it = tmap.emplace(key, (int)tmap.size()).first;
+    ty[i] = it->second;
+  }
+  const int nt = (int)tmap.size();
+  *ntype_out = nt;
+  if (nt > maxtypes) {
+    set_last_error("mm_param_gradients: more site types than maxtypes");
+    return MM_ERR_ARG;
+  }
+  std::vector<double> Q(nt), sg(nt), ep(nt);
+  std::vector<int> haslj(nt);
+  for (auto &kv : tmap) {
+    const int t = kv.second;
+    Q[t] = std::get<0>(kv.first);
+    sg[t] = std::get<1>(kv.first);
+    ep[t] = std::get<2>(kv.first);
+    haslj[t] = (std::get<3>(kv.first) & SITE_HAS_LJ) ? 1 : 0;
+  }
+  std::map<std::pair<double, double>, int> ljtypes;   // engine LJ types (engine.cpp)
+  for (int i = 0; i < N; i++)
+    if (S.mask[i] & SITE_HAS_LJ) {
+      auto key = std::make_pair(S.sig[i], S.eps[i]);
+      if (!ljtypes.count(key)) ljtypes.emplace(key, (int)ljtypes.size());
+    }
+  const int nlj = P.ntype;
+  std::vector<int> owner(nlj, -1);
+  for (int t = 0; t < nt; t++)
+    if (haslj[t]) {
+      const int lt = ljtypes.at(std::make_pair(sg[t], ep[t]));
+      if (owner[lt] >= 0) {
+        set_last_error("mm_param_gradients: two site types share LJ parameters but differ in charge");
+        return MM_ERR_ARG;
+      }
+      owner[lt] = t;
+    }
+  if (site_type)
+    for (int i = 0; i < N; i++) site_type[i] = ty[i];
+  if (type_params)
+    for (int t = 0; t < nt; t++) {
+      type_params[3 * t] = Q[t];
+      type_params[3 * t + 1] = sg[t];
+      type_params[3 * t + 2] = ep[t];
+    }
+
+  // forward evaluation: forces, potentials, reciprocal-space fields
+  if (int rc = handle_compute(m)) return rc;
+  std::vector<double> f0, phi0;
+  if (int rc = evaluate(m, sys, f0, nullptr, &phi0)) return rc;
+  std::vector<double> kres(4 * (size_t)N);
+  MM_HIP(hipMemcpy(kres.data(), e.d_kres + 4L * P.site0, sizeof(double) * kres.size(),
+                   hipMemcpyDeviceToHost));
+  // R = 2 (F - F_ref), SoA over all sites of the engine (zero elsewhere)
+  std::vector<double> R(3 * (size_t)N, 0.0), Rsoa(3 * (size_t)NT, 0.0);
+  double L = 0;
+  if (f_ref)
+    for (int i = 0; i < N; i++)
+      for (int c = 0; c < 3; c++) {
+        const double d = f0[3 * i + c] - f_ref[3 * i + c];
+        L += d * d;
+        R[3 * i + c] = 2 * d;
+        Rsoa[(size_t)c * NT + P.site0 + i] = 2 * d;
+      }
+  if (loss) *loss = L;
+  const int nblk = adj_blocks(e, sys), nslot = adj_part_slots();
+  TRY_ADJ(ensure(&e.d_adjR, 3 * (size_t)NT));
+  TRY_ADJ(ensure(&e.d_chi, (size_t)NT));
+  TRY_ADJ(ensure(&e.d_adjpart, (size_t)std::max(e.ngroup, 1) * nslot));
+  TRY_ADJ(ensure(&e.d_adjksc, 2 * (size_t)std::max(e.nk_total, 1)));
+  TRY_ADJ(ensure(&e.d_adjkres, 4 * (size_t)NT + 16));
+  TRY_ADJ(ensure(&e.d_adjacc, (size_t)NACC));
+  TRY_ADJ(ensure(&e.d_adjsrc, (size_t)NT));
+  MM_HIP(hipMemcpy(e.d_adjR, Rsoa.data(), sizeof(double) * Rsoa.size(), hipMemcpyHostToDevice));
+  // P3M: the adjoint source is scaled to at most 1 per site so that the
+  // spreading's fixed-point sums keep the forward's range
+  double scale = 1.0;
+  if (P.kmode == 1 && f_ref) {
+    double mx = 0;
+    for (int i = 0; i < N; i++)
+      for (int d = 0; d < 3; d++) {
+        double v = 0;
+        for (int c = 0; c < 3; c++) v += R[3 * i + c] * P.T[3 * c + d];
+        mx = std::max(mx, fabs(S.q[i] * v));
+      }
+    scale = mx > 0 ? 1.0 / mx : 1.0;
+  }
+  hipEvent_t t0, t1;
+  MM_HIP(hipEventCreate(&t0));
+  MM_HIP(hipEventCreate(&t1));
+  MM_HIP(hipEventRecord(t0, e.stream));
+  int rc = launch_adj_pair(e, sys, e.d_adjR, e.d_chi, e.d_adjpart);
+  if (!rc && f_ref && P.kmode == 0)
+    rc = launch_adj_ewald(e, sys, e.d_adjR, e.d_adjksc, e.d_adjkres, e.d_adjacc);
+  if (!rc && f_ref && P.kmode == 1) {
+    rc = launch_adj_p3m_src(e, sys, e.d_adjR, scale, e.d_adjsrc);
+    if (!rc) rc = launch_p3m_adjoint(e, sys, e.d_adjsrc, e.stream);
+  }
+  MM_HIP(hipEventRecord(t1, e.stream));
+  MM_HIP(hipEventSynchronize(t1));
+  float tms = 0;
+  MM_HIP(hipEventElapsedTime(&tms, t0, t1));
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  // the reciprocal-space buffers now hold the adjoint: the next evaluation
+  // starts from scratch
+  m->pos_dirty = 1;
+  if (rc) return rc;
+  if (ms) *ms = tms;
+  std::vector<double> chi(N), part((size_t)nblk * nslot), psi(4 * (size_t)N, 0.0);
+  MM_HIP(hipMemcpy(chi.data(), e.d_chi + P.site0, sizeof(double) * N, hipMemcpyDeviceToHost));
+  MM_HIP(hipMemcpy(part.data(), e.d_adjpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
+  if (f_ref && P.kmode == 0)
+    MM_HIP(hipMemcpy(psi.data(), e.d_adjkres + 4L * P.site0, sizeof(double) * psi.size(),
+                     hipMemcpyDeviceToHost));
+  if (f_ref && P.kmode == 1)
+    MM_HIP(hipMemcpy(psi.data(), e.d_kres + 4L * P.site0, sizeof(double) * psi.size(),
+                     hipMemcpyDeviceToHost));
+  {
+    int flags = 0;
+    MM_HIP(hipMemcpy(&flags, e.d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (flags & ERR_BIN_OVERFLOW) {   // cannot happen: the forward grew the bins
+      set_last_error("mm_param_gradients: P3M bin overflow in the adjoint pass");
+      return MM_ERR_DEVICE;
+    }
+  }
+  std::vector<double> gE(3 * nt, 0.0), gL(3 * nt, 0.0);
+  for (int i = 0; i < N; i++) {
+    gE[3 * ty[i]] += phi0[i];   // dE/dq_i = phi_i
+    if (!f_ref) continue;
+    // chi_i: pair and excluded terms, R_i . E_i and psi_i of the mesh or k sum
+    const double *E = &kres[4 * (size_t)i];
+    const double rdotE = R[3 * i] * E[1] + R[3 * i + 1] * E[2] + R[3 * i + 2] * E[3];
+    gL[3 * ty[i]] += chi[i] + (P.kmode <= 1 ? rdotE + psi[4 * (size_t)i] / scale : 0.0);
+  }
+  // LJ: per engine type pair, (dL/da, dL/db, dE/da, dE/db) summed over the
+  // blocks in block order, both orientations of an unordered pair together
+  std::vector<double> lj(8 * 8 * 4, 0.0);
+  for (int b = 0; b < nblk; b++)
+    for (int k = 0; k < nslot; k++) lj[k] += part[(size_t)b * nslot + k];
+  double ca = 0, cb = 0;
+  if (S.include_lj_correction) lj_tail_factors(S, &ca, &cb);
+  std::vector<double> nsites(nt, 0.0);
+  for (int i = 0; i < N; i++)
+    if (S.mask[i] & SITE_HAS_LJ) nsites[ty[i]] += 1;
+  for (int lt = 0; lt < nlj; lt++)
+    for (int lu = lt; lu < nlj; lu++) {
+      const int t = owner[lt], u = owner[lu];
+      if (t < 0 || u < 0) continue;
+      double d[4];
+      for (int c = 0; c < 4; c++)
+        d[c] = lj[(lt * 8 + lu) * 4 + c] + (lt != lu ? lj[(lu * 8 + lt) * 4 + c] : 0.0);
+      const double e_tu = sqrt(ep[t] * ep[u]);
+      const double s_tu = S.geometric_sigma ? sqrt(sg[t] * sg[u]) : 0.5 * (sg[t] + sg[u]);
+      const double s6 = pow(s_tu, 6);
+      const double a = 4 * e_tu * s6 * s6, b = 4 * e_tu * s6;
+      double de[2], ds[2];
+      const int who[2] = {t, u};
+      for (int k = 0; k < 2; k++) {
+        const int w = who[k], o = who[1 - k];
+        de[k] = (t == u) ? 0.5 : 0.5 * (ep[w] > 0 ? sqrt(ep[o] / ep[w]) : 0.0);
+        ds[k] = S.geometric_sigma ? ((t == u) ? 0.5 : 0.5 * sqrt(sg[o] / sg[w])) : 0.5;
+      }
+      const double mult = (lt == lu) ? 1.0 : 2.0;
+      const double ntail = nsites[t] * nsites[u] * mult;
+      for (int coef = 0; coef < 2; coef++) {
+        // tail: U = ca totA + cb totB, totA = sum n n a, totB = -sum n n b
+        const double dEdc = d[2 + coef] + (coef == 0 ? ca * ntail : -cb * ntail);
+        const double dLdc = f_ref ? d[coef] : 0.0;
+        const double cval = coef == 0 ? a : b;
+        const double p = coef == 0 ? 12.0 : 6.0;   // c = 4 e s^p
+        for (int k = 0; k < 2; k++) {
+          const int w = who[k];
+          const double dc_ds = cval * p / s_tu * ds[k];
+          const double dc_de = (e_tu > 0) ? cval / e_tu * de[k] : 0.0;
+          gE[3 * w + 1] += dEdc * dc_ds;
+          gE[3 * w + 2] += dEdc * dc_de;
+          gL[3 * w + 1] += dLdc * dc_ds;
+          gL[3 * w + 2] += dLdc * dc_de;
+        }
+      }
+    }
+  if (dE)
+    for (int k = 0; k < 3 * nt; k++) dE[k] = gE[k];
+  if (dL)
+    for (int k = 0; k < 3 * nt; k++) dL[k] = gL[k];
+  return MM_OK;
+}
+
+// ------------------------------------------------------------ parameters
+extern "C" int mm_set_site_params(mm_engine *m, int sys, const double *q, const double *sigma,
+                                  const double *epsilon) {
+  if (!m || sys < 0 || sys >= m->e.nsys || !q || !sigma || !epsilon) {
+    set_last_error("mm_set_site_params: bad arguments");
+    return MM_ERR_ARG;
+  }
+  if (int rc = engine_set_params(m->e, sys, q, sigma, epsilon)) return rc;
+  m->pos_dirty = 1;   // forces of the old parameters are stale
+  return MM_OK;
+}
+
+extern "C" int mm_set_perturbation(mm_engine *m, int sys, const double *qB, const double *sigB,
+                                   const double *epsB) {
+  if (!m || sys < 0 || sys >= m->e.nsys || !qB || !sigB || !epsB) {
+    set_last_error("mm_set_perturbation: bad arguments");
+    return MM_ERR_ARG;
+  }
+  Engine &e = m->e;
+  const SystemInput &S = e.in[sys];
+  const size_t n = S.nsite;
+  if (e.pert.size() != (size_t)e.nsys) e.pert.resize(e.nsys);
+  std::vector<double> &p = e.pert[sys];
+  p.assign(6 * n, 0.0);
+  for (size_t i = 0; i < n; i++) {
+    p[i] = S.q[i];
+    p[n + i] = S.sig[i];
+    p[2 * n + i] = S.eps[i];
+    p[3 * n + i] = qB[i];
+    p[4 * n + i] = sigB[i];
+    p[5 * n + i] = epsB[i];
+  }
+  return MM_OK;
+}
+
+// ElecLJ::set_lambda (eleclj.cpp:273-308, Site::set_lambda in eleclj.hpp):
+// q = (1 - l) qA + l qB for every site; sigma and epsilon likewise for the
+// sites with LJ (the others keep theirs); then the LJ types, tables and the
+// tail correction over the "%12.6g" keys of the new parameters.
+extern "C" int mm_set_lambda(mm_engine *m, int sys, double lambda) {
+  if (!m || sys < 0 || sys >= m->e.nsys || !std::isfinite(lambda)) {
+    set_last_error("mm_set_lambda: bad arguments");
+    return MM_ERR_ARG;
+  }
+  Engine &e = m->e;
+  if (e.pert.size() != (size_t)e.nsys || e.pert[sys].empty()) {
+    set_last_error("mm_set_lambda: mm_set_perturbation first");
+    return MM_ERR_ARG;
+  }
+  const SystemInput &S = e.in[sys];
+  const size_t n = S.nsite;
+  const std::vector<double> &p = e.pert[sys];
+  const double tmp = 1 - lambda;
+  std::vector<double> q(n), sg(n), ep(n);
+  for (size_t i = 0; i < n; i++) {
+    q[i] = tmp * p[i] + lambda * p[3 * n + i];
+    if (S.mask[i] & SITE_HAS_LJ) {
+      sg[i] = tmp * p[n + i] + lambda * p[4 * n + i];
+      ep[i] = tmp * p[2 * n + i] + lambda * p[5 * n + i];
+    } else {
+      sg[i] = S.sig[i];
+      ep[i] = S.eps[i];
+    }
+  }
+  return mm_set_site_params(m, sys, q.data(), sg.data(), ep.data());
 }

@@ -221,6 +221,54 @@ static int dput(T *d, const std::vector<T> &h) {
     if (rc_) return rc_;   \
   } while (0)
 
+// LJ types of one system: distinct (sigma, epsilon) among its LJ sites,
+// numbered in order of first appearance (stype per site, 0 for sites
+// without LJ), and the type-pair table (b, a) with Lorentz-Berthelot or
+// geometric mixing (rspace.c:267-270) appended to ljt.
+static int lj_types(const SystemInput &S, std::vector<int> &stype, std::vector<float2> &ljt,
+                    int &ntype) {
+  std::map<std::pair<double, double>, int> types;
+  stype.assign(S.nsite, 0);
+  for (int i = 0; i < S.nsite; i++)
+    if (S.mask[i] & SITE_HAS_LJ) {
+      auto key = std::make_pair(S.sig[i], S.eps[i]);
+      auto it = types.find(key);
+      if (it == types.end()) it = types.emplace(key, (int)types.size()).first;
+      stype[i] = it->second;
+    }
+  const int nt = std::max(1, (int)types.size());
+  MM_REQ(nt <= 8, "at most 8 distinct LJ parameter pairs per system on the device");
+  std::vector<std::pair<double, double>> tp(nt, {0.0, 0.0});
+  for (auto &t : types) tp[t.second] = t.first;
+  for (int a = 0; a < nt; a++)
+    for (int b = 0; b < nt; b++) {
+      const double s2 = S.geometric_sigma ? tp[a].first * tp[b].first
+                                          : sq(0.5 * (tp[a].first + tp[b].first));
+      const double s6 = s2 * s2 * s2;
+      const double bb = 4 * sqrt(tp[a].second * tp[b].second) * s6;
+      ljt.push_back(make_float2((float)bb, (float)(bb * s6)));
+    }
+  ntype = nt;
+  return 0;
+}
+
+// per-group packed (mask | type << 2) bytes of one system's sites
+static int group_info(const SystemInput &S, const std::vector<int> &stype, int g) {
+  int info = 0;
+  for (int k = 0; k < S.gns[g]; k++) {
+    const int i = S.gfirst[g] + k;
+    info |= ((S.mask[i] & 3) | (stype[i] << 2)) << (8 * k);
+  }
+  return info;
+}
+
+static double background_phi(const SystemInput &S, double eta, double volume) {
+  double qsum = 0;
+  for (double x : S.q) qsum += x;
+  // eleclj.cpp:970-975
+  return fabs(qsum) < 64.0 * 2.220446049250313e-16 ? 0.0 : -M_PI / (eta * eta * volume) * qsum;
+}
+
 // The tail energy is linear in totA = sum n_a n_b a_ab and
 // totB = -sum n_a n_b b_ab: U = ca totA + cb totB (eleclj.cpp:1259-1264)
 void lj_tail_factors(const SystemInput &s, double *ca, double *cb) {
@@ -326,30 +374,9 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
       }
     }
     for (int i = 0; i < S.nsite; i++) MM_REQ(gof[i] >= 0, "site not in any group");
-    // LJ types: distinct (sigma, epsilon) among LJ sites
-    std::map<std::pair<double, double>, int> types;
-    std::vector<int> stype(S.nsite, 0);
-    for (int i = 0; i < S.nsite; i++)
-      if (S.mask[i] & SITE_HAS_LJ) {
-        auto key = std::make_pair(S.sig[i], S.eps[i]);
-        auto it = types.find(key);
-        if (it == types.end()) it = types.emplace(key, (int)types.size()).first;
-        stype[i] = it->second;
-      }
-    const int nt = std::max(1, (int)types.size());
-    MM_REQ(nt <= 8, "at most 8 distinct LJ parameter pairs per system on the device");
-    std::vector<std::pair<double, double>> tp(nt, {0.0, 0.0});
-    for (auto &t : types) tp[t.second] = t.first;
-    P.ntype = nt;
+    std::vector<int> stype;
     P.lj0 = (int)ljt.size();
-    for (int a = 0; a < nt; a++)
-      for (int b = 0; b < nt; b++) {   // rspace.c:267-270
-        const double s2 = S.geometric_sigma ? tp[a].first * tp[b].first
-                                            : sq(0.5 * (tp[a].first + tp[b].first));
-        const double s6 = s2 * s2 * s2;
-        const double bb = 4 * sqrt(tp[a].second * tp[b].second) * s6;
-        ljt.push_back(make_float2((float)bb, (float)(bb * s6)));
-      }
+    MM_TRY(lj_types(S, stype, ljt, P.ntype));
     // exclusions: intra-group only on the device
     std::vector<int> gx(S.ngroup, 0);
     for (size_t k = 0; 2 * k + 1 < S.excl.size(); k++) {
@@ -367,21 +394,14 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
       gns.push_back(S.gns[g]);
       gsys.push_back(s);
       gexcl.push_back(gx[g]);
-      int info = 0;
-      for (int k = 0; k < S.gns[g]; k++) {
-        const int i = S.gfirst[g] + k;
-        info |= ((S.mask[i] & 3) | (stype[i] << 2)) << (8 * k);
-      }
-      ginfo.push_back(info);
+      ginfo.push_back(group_info(S, stype, g));
     }
-    double qsum = 0;
     for (int i = 0; i < S.nsite; i++) {
       q.push_back(S.q[i]);
-      qsum += S.q[i];
       mass.push_back(S.mass.empty() ? 1.0 : S.mass[i]);
     }
     // neutralising background (eleclj.cpp:970-975)
-    P.bg_phi = fabs(qsum) < 64.0 * 2.220446049250313e-16 ? 0.0 : -M_PI / (sq(S.eta) * P.volume) * qsum;
+    P.bg_phi = background_phi(S, S.eta, P.volume);
     // neighbor list
     P.ncell = nl_ncell(S.ngroup);
     P.lc = S.boxl * (1.0 / P.ncell);
@@ -691,6 +711,57 @@ int engine_scale_system(Engine &e, int s, double f) {
   return 0;
 }
 
+// New charges and LJ parameters for the sites of system s (site masks,
+// groups and exclusions stay): LJ types and the type-pair table of every
+// system are rebuilt (a system's type count may change), the tail
+// correction and the neutralising background follow, and the next
+// evaluation packs the new charges.  This is what ElecLJ::set_lambda
+// leaves behind (eleclj.cpp:273-308) for a new set of site parameters.
+int engine_set_params(Engine &e, int s, const double *q, const double *sig, const double *eps) {
+  MM_REQ(s >= 0 && s < e.nsys, "bad system index");
+  SystemInput &S = e.in[s];
+  for (int i = 0; i < S.nsite; i++)
+    MM_REQ(std::isfinite(q[i]) && std::isfinite(sig[i]) && std::isfinite(eps[i]) && eps[i] >= 0,
+           "site parameters must be finite, epsilon >= 0");
+  const SystemInput keep = S;
+  S.q.assign(q, q + S.nsite);
+  S.sig.assign(sig, sig + S.nsite);
+  S.eps.assign(eps, eps + S.nsite);
+  std::vector<float2> ljt;
+  std::vector<int> ntype(e.nsys), lj0(e.nsys), stype_s;
+  for (int t = 0; t < e.nsys; t++) {
+    std::vector<int> st;
+    lj0[t] = (int)ljt.size();
+    if (int rc = lj_types(e.in[t], st, ljt, ntype[t])) {
+      S = keep;
+      return rc;
+    }
+    if (t == s) stype_s = st;
+  }
+  for (int t = 0; t < e.nsys; t++) {
+    e.hsp[t].ntype = ntype[t];
+    e.hsp[t].lj0 = lj0[t];
+  }
+  SysParams &P = e.hsp[s];
+  std::vector<int> ginfo(S.ngroup);
+  for (int g = 0; g < S.ngroup; g++) ginfo[g] = group_info(S, stype_s, g);
+  MM_HIP(hipStreamSynchronize(e.stream));
+  MM_HIP(hipMemcpy(e.d_ginfo + P.group0, ginfo.data(), sizeof(int) * S.ngroup, hipMemcpyHostToDevice));
+  MM_HIP(hipMemcpy(e.d_q + P.site0, S.q.data(), sizeof(double) * S.nsite, hipMemcpyHostToDevice));
+  MM_HIP(hipFree(e.d_ljt));
+  e.d_ljt = nullptr;
+  MM_TRY(dalloc(&e.d_ljt, ljt.size()));
+  MM_TRY(dput(e.d_ljt, ljt));
+  P.bg_phi = background_phi(S, S.eta, P.volume);
+  e.lj_tail[s] = S.include_lj_correction ? lj_tail(S) : 0.0;
+  if (e.d_tail)
+    MM_HIP(hipMemcpy(e.d_tail, e.lj_tail.data(), sizeof(double) * e.nsys, hipMemcpyHostToDevice));
+  MM_TRY(dput(e.d_sp, e.hsp));
+  e.need_pack = 1;
+  engine_drop_graph(e);   // the table may have moved
+  return 0;
+}
+
 void engine_drop_graph(Engine &e) {
   if (e.gexec) { hipGraphExecDestroy(e.gexec); e.gexec = nullptr; }
   if (e.graph) { hipGraphDestroy(e.graph); e.graph = nullptr; }
@@ -733,7 +804,9 @@ void engine_destroy(Engine &e) {
                   e.d_sgrid, e.d_sfrac, e.d_bin_fill,
                   e.d_bgrid, e.d_bfrac, e.d_acc, e.d_part, e.d_fpart, e.d_done, e.d_phi, e.d_scratch,
                   e.d_nlb, e.d_ladder, e.d_kT, e.d_tail, e.d_rtmp, e.d_mom, e.d_perm,
-                  e.d_remd_stats, e.d_tile_start, e.d_snap, e.d_snap_perm, e.d_snap_stats};
+                  e.d_remd_stats, e.d_tile_start, e.d_snap, e.d_snap_perm, e.d_snap_stats,
+                  e.d_adjR, e.d_chi, e.d_adjpart, e.d_adjksc, e.d_adjkres, e.d_adjacc,
+                  e.d_adjsrc};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (e.stream) hipStreamDestroy(e.stream);

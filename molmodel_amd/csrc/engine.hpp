@@ -150,6 +150,14 @@ struct Engine {
   long long *d_remd_stats = nullptr;
   unsigned long long remd_seed = 0;
   long long remd_attempt = 0;
+  // per system, the two parameter sets of a perturbation (ElecLJ's
+  // types_changed(pert), eleclj.cpp:273-308): q, sigma, epsilon of state A
+  // then of state B (6 nsite), empty when none is set
+  std::vector<std::vector<double>> pert;
+  // adjoint buffers (allocated by the first gradient call)
+  double *d_adjR = nullptr, *d_chi = nullptr, *d_adjpart = nullptr, *d_adjksc = nullptr;
+  double *d_adjkres = nullptr, *d_adjacc = nullptr;
+  float4 *d_adjsrc = nullptr;
   // state saved at the start of an MD block or exchange interval, restored
   // when an overflow makes the block run again (r, v: 6 N; exchange
   // permutation and statistics)
@@ -171,6 +179,9 @@ void cheb_fit(double a, double b, double (*f)(double), double *c4);
 void engine_destroy(Engine &e);
 void lj_tail_factors(const SystemInput &s, double *ca, double *cb);
 int engine_scale_system(Engine &e, int s, double f);
+// new site charges and LJ parameters for system s (ElecLJ::set_lambda's
+// effect, eleclj.cpp:273-308): LJ types, tables, tail and background follow
+int engine_set_params(Engine &e, int s, const double *q, const double *sig, const double *eps);
 // drop the captured step graph (it holds buffer addresses that changed)
 void engine_drop_graph(Engine &e);
 // after ERR_BIN_OVERFLOW: grow the P3M column bins to the capacity the
@@ -194,6 +205,21 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies = 1);
 inline int launch_kspace(Engine &e, int want_virial) { return launch_kspace(e, want_virial, e.stream); }
 inline int launch_p3m(Engine &e, int want_virial) { return launch_p3m(e, want_virial, e.stream); }
 int launch_finish(Engine &e, int mode);
+// k-space structure factors and field of one system in buffers the caller
+// provides (kspace.hip)
+int launch_kspace_raw(const SysParams *sp, const double4 *kvec, int nk, int nsite,
+                      const double *r, const double *q, double *ksc, double *kres, double *acc,
+                      hipStream_t st);
+// adjoint passes of one system (adjoint.hip, kspace.hip, pppm.hip)
+int launch_kspace_field_only(Engine &e, int sys, const double *ksc, double *kres_out,
+                             double *acc_out);
+int launch_p3m_adjoint(Engine &e, int sys, const float4 *src, hipStream_t st);
+int adj_part_slots();
+int adj_blocks(const Engine &e, int sys);
+int launch_adj_pair(Engine &e, int sys, const double *R, double *chi, double *part);
+int launch_adj_ewald(Engine &e, int sys, const double *R, double *ksc_adj, double *kres_adj,
+                     double *acc_scratch);
+int launch_adj_p3m_src(Engine &e, int sys, const double *R, double scale, float4 *src);
 int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out);
 int launch_field_out(Engine &e, double *out, double *ulj, int min_image = 1);
 int launch_skin(Engine &e);

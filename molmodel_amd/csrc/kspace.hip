@@ -140,4 +140,30 @@ int launch_kspace(Engine &e, int want_virial, hipStream_t st) {
   return 0;
 }
 
+// Structure factors and field of one system whose parameters, wave vectors,
+// SoA positions and result buffers the caller provides (kspace_field_noalloc
+// keeps them in its own device array): ksc 2 nk, kres 4 nsite + 9, acc NACC.
+int launch_kspace_raw(const SysParams *sp, const double4 *kvec, int nk, int nsite,
+                      const double *r, const double *q, double *ksc, double *kres, double *acc,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_ks_sf, dim3(nk, 1), dim3(256), 0, st, sp, kvec, r, q, nsite, ksc);
+  hipLaunchKernelGGL(k_ks_field, dim3((nsite + KS_SITES - 1) / KS_SITES, 1), dim3(KS_TB), 0, st,
+                     sp, kvec, ksc, r, nsite, kres, acc, 0);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+// the field kernel of one system with other structure factors and outputs
+// (the adjoint pass): kres_out needs 4 N + 9 entries, acc_out NACC
+int launch_kspace_field_only(Engine &e, int sys, const double *ksc, double *kres_out,
+                             double *acc_out) {
+  const SysParams &P = e.hsp[sys];
+  if (P.kmode != 0) return 0;
+  hipLaunchKernelGGL(k_ks_field, dim3((P.nsite + KS_SITES - 1) / KS_SITES, 1), dim3(KS_TB), 0,
+                     e.stream, e.d_sp + sys, (const double4 *)e.d_kvec, ksc, e.d_r, e.nsite,
+                     kres_out, acc_out, 0);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace mmamd

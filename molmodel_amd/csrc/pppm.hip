@@ -141,12 +141,19 @@ __device__ __forceinline__ int floor_div(int a, int b) { return a >= 0 ? a / b :
 // within SPREAD_MAXPTS, see choose_spread_blocks)
 constexpr int SPREAD_MAXPTS = 4096;
 
-template <int P>
+// ADJ: the transpose of the gather's field instead of the charge: site i
+// contributes src_i . (dw_x w_y w_z, w_x dw_y w_z, w_x w_y dw_z) with dw the
+// negated weight derivatives the gather uses and src_i = q_i (R_i T) (the
+// adjoint of E_i = T^T sum_m dw phi_m for a given R_i), so that the
+// convolution and an ordinary gather of the result give
+// sum_i q_i R_i . dE_i/dq_j at every site j.
+template <int P, bool ADJ = false>
 __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict__ sp,
                                                     const int *__restrict__ fill, int bcap,
                                                     const int4 *__restrict__ bgrid,
                                                     const float4 *__restrict__ bfrac,
-                                                    float *__restrict__ mesh) {
+                                                    float *__restrict__ mesh,
+                                                    const float4 *__restrict__ src = nullptr) {
   __shared__ unsigned long long acc[SPREAD_MAXPTS];
   const int s = blockIdx.y;
   const SysParams &Q = sp[s];
@@ -199,10 +206,10 @@ __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict_
     const int t = sb_start[lo] + (u - sb_off[lo]);
     const int4 g = bgrid[t];
     const float4 f = bfrac[t];
-    float wx[P], wy[P], wz[P];
-    bspline_w<P>(f.x, wx, nullptr);
-    bspline_w<P>(f.y, wy, nullptr);
-    bspline_w<P>(f.z, wz, nullptr);
+    float wx[P], wy[P], wz[P], dwx[P], dwy[P], dwz[P];
+    bspline_w<P>(f.x, wx, ADJ ? dwx : nullptr);
+    bspline_w<P>(f.y, wy, ADJ ? dwy : nullptr);
+    bspline_w<P>(f.z, wz, ADJ ? dwz : nullptr);
     int lx[P], ly[P], iz[P];
 #pragma unroll
     for (int a = 0; a < P; a++) {
@@ -210,6 +217,8 @@ __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict_
       ly[a] = wrap(g.y + istart + a, ny) - Y0;
       iz[a] = wrap(g.z + istart + a, nz);
     }
+    float4 sv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ADJ) sv = src[g.w];
 #pragma unroll
     for (int a = 0; a < P; a++) {
       if (lx[a] < 0 || lx[a] >= bx) continue;
@@ -217,11 +226,20 @@ __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict_
 #pragma unroll
       for (int bb = 0; bb < P; bb++) {
         if (ly[bb] < 0 || ly[bb] >= by) continue;
-        const float qw = qa * wy[bb];
         unsigned long long *col = acc + (lx[a] * by + ly[bb]) * nz;
+        if (ADJ) {
+          const float al = sv.x * dwx[a] * wy[bb] + sv.y * wx[a] * dwy[bb];
+          const float be = sv.z * wx[a] * wy[bb];
 #pragma unroll
-        for (int c = 0; c < P; c++)
-          atomicAdd(col + iz[c], (unsigned long long)__float2ll_rn(qw * wz[c] * FIX_SCALE));
+          for (int c = 0; c < P; c++)
+            atomicAdd(col + iz[c],
+                      (unsigned long long)__float2ll_rn((al * wz[c] + be * dwz[c]) * FIX_SCALE));
+        } else {
+          const float qw = qa * wy[bb];
+#pragma unroll
+          for (int c = 0; c < P; c++)
+            atomicAdd(col + iz[c], (unsigned long long)__float2ll_rn(qw * wz[c] * FIX_SCALE));
+        }
       }
     }
   }
@@ -522,29 +540,31 @@ __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict_
   kres[4 * si + 3] = (double)e0 * T[6] + (double)e1 * T[7] + (double)e2 * T[8];
 }
 
-// forward z, y passes; x pass with the influence function; inverse y, z
+// forward z, y passes; x pass with the influence function; inverse y, z.
+// sp / nsg: the systems the grid's y extent covers (all of them, or one
+// system's parameters with nsg = 1)
 template <bool POW2>
 static void launch_fft(Engine &e, int want_virial, int energies, hipStream_t st, int maxx,
-                       int maxy, int maxz) {
-  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+                       int maxy, int maxz, const SysParams *sp, int nsg) {
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, nsg), dim3(FFT_TB), 0, st, sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 0);
-  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, nsg), dim3(FFT_TB), 0, st, sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 1);
   if (want_virial)
-    hipLaunchKernelGGL((k_fft_xgreen<true, POW2>), dim3(maxx, e.nsys), dim3(FFT_TB), 0, st,
-                       e.d_sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
+    hipLaunchKernelGGL((k_fft_xgreen<true, POW2>), dim3(maxx, nsg), dim3(FFT_TB), 0, st,
+                       sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
                        e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
   else if (energies)
-    hipLaunchKernelGGL((k_fft_xgreen<false, POW2>), dim3(maxx, e.nsys), dim3(FFT_TB), 0, st,
-                       e.d_sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
+    hipLaunchKernelGGL((k_fft_xgreen<false, POW2>), dim3(maxx, nsg), dim3(FFT_TB), 0, st,
+                       sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
                        e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
   else
-    hipLaunchKernelGGL((k_fft_xgreen<false, POW2, false>), dim3(maxx, e.nsys), dim3(FFT_TB), 0,
-                       st, e.d_sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
+    hipLaunchKernelGGL((k_fft_xgreen<false, POW2, false>), dim3(maxx, nsg), dim3(FFT_TB), 0,
+                       st, sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
                        e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
-  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, nsg), dim3(FFT_TB), 0, st, sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 3);
-  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, e.nsys), dim3(FFT_TB), 0, st, e.d_sp,
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, nsg), dim3(FFT_TB), 0, st, sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 4);
 }
 
@@ -580,14 +600,67 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies) {
   for (auto &p : e.hsp)
     if (p.kmode == 1)
       for (int n : {p.nx, p.ny, p.nz}) pow2 &= (n & (n - 1)) == 0;
-  if (pow2) launch_fft<true>(e, want_virial, energies, st, maxx, maxy, maxz);
-  else launch_fft<false>(e, want_virial, energies, st, maxx, maxy, maxz);
+  if (pow2) launch_fft<true>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys);
+  else launch_fft<false>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys);
 #define GATHER(PO)                                                                        \
   if (used[PO])                                                                           \
     hipLaunchKernelGGL(k_p3m_gather<PO>, gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid,     \
                        e.d_sfrac, e.d_mesh, e.d_kres, e.d_bin_fill);
   GATHER(1) GATHER(2) GATHER(3) GATHER(4) GATHER(5) GATHER(6) GATHER(7)
 #undef GATHER
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+// The mesh adjoint of one P3M system: bin its sites again (the last gather
+// cleared the counters; positions are those of the last evaluation), spread
+// the adjoint source src (per site, global site index), convolve with the
+// influence function (no energy), and gather the potential of the result
+// into kres[4 i] of the system's sites (kres[4 i + 1..3] receive its
+// field, unused).
+int launch_p3m_adjoint(Engine &e, int sys, const float4 *src, hipStream_t st) {
+  const SysParams &P = e.hsp[sys];
+  if (P.kmode != 1) return 0;
+  const SysParams *sp = e.d_sp + sys;
+  const int tb = 256;
+  const dim3 gs((P.nsite + tb - 1) / tb, 1);
+  hipLaunchKernelGGL(k_p3m_bin, gs, dim3(tb), 0, st, sp, e.d_r, e.d_q, e.nsite, e.d_sgrid,
+                     e.d_sfrac, e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_err,
+                     e.d_bin_fill + e.nbin_total);
+  const dim3 gb(P.nbx * P.nby, 1);
+#define ADJ_ORDER(PO)                                                                        \
+  case PO:                                                                                   \
+    hipLaunchKernelGGL((k_p3m_spread<PO, true>), gb, dim3(256), 0, st, sp, e.d_bin_fill,     \
+                       e.bcap, e.d_bgrid, e.d_bfrac, e.d_mesh, src);                         \
+    break;
+  switch (P.order) {
+    ADJ_ORDER(1) ADJ_ORDER(2) ADJ_ORDER(3) ADJ_ORDER(4) ADJ_ORDER(5) ADJ_ORDER(6) ADJ_ORDER(7)
+    default: set_last_error("assignment order out of range"); return -1;
+  }
+#undef ADJ_ORDER
+  const int nzh = P.nz / 2 + 1;
+  const int maxz = (P.nx * P.ny + FFT_LINES - 1) / FFT_LINES;
+  const int maxy = (P.nx * nzh + FFT_LINES - 1) / FFT_LINES;
+  const int maxx = (P.ny * nzh + FFT_LINES - 1) / FFT_LINES;
+  bool pow2 = true;
+  for (int n : {P.nx, P.ny, P.nz}) pow2 &= (n & (n - 1)) == 0;
+  // the engine chose one FFT build for all its meshes; keep to it
+  for (auto &p : e.hsp)
+    if (p.kmode == 1)
+      for (int n : {p.nx, p.ny, p.nz}) pow2 &= (n & (n - 1)) == 0;
+  if (pow2) launch_fft<true>(e, 0, 0, st, maxx, maxy, maxz, sp, 1);
+  else launch_fft<false>(e, 0, 0, st, maxx, maxy, maxz, sp, 1);
+#define ADJ_GATHER(PO)                                                                      \
+  case PO:                                                                                  \
+    hipLaunchKernelGGL(k_p3m_gather<PO>, gs, dim3(tb), 0, st, sp, e.d_sgrid, e.d_sfrac,     \
+                       e.d_mesh, e.d_kres, e.d_bin_fill);                                   \
+    break;
+  switch (P.order) {
+    ADJ_GATHER(1) ADJ_GATHER(2) ADJ_GATHER(3) ADJ_GATHER(4) ADJ_GATHER(5) ADJ_GATHER(6)
+    ADJ_GATHER(7)
+    default: break;
+  }
+#undef ADJ_GATHER
   MM_HIP(hipGetLastError());
   return 0;
 }

@@ -69,6 +69,10 @@ def lib():
         L.mm_remd_stats.argtypes = [_P, _P, _P]
         L.mm_tune_kspace.argtypes = [_D, _I, _D, _D, _I, _P]
         L.mm_param_gradients.argtypes = [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
+        L.mm_param_gradients_basis.argtypes = [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
+        L.mm_set_site_params.argtypes = [_P, _I, _P, _P, _P]
+        L.mm_set_perturbation.argtypes = [_P, _I, _P, _P, _P]
+        L.mm_set_lambda.argtypes = [_P, _I, _D]
         L.mm_scale_system.argtypes = [_P, _I, _D]
         L.mm_nl_builds.restype = C.c_longlong
         L.mm_nl_builds.argtypes = [_P]
@@ -252,9 +256,12 @@ class Engine:
         return _check(lib().mm_nl_builds(self.h))
 
     # parameter gradients -----------------------------------------------------
-    def param_gradients(self, sys=0, f_ref=None, maxtypes=16):
+    def param_gradients(self, sys=0, f_ref=None, maxtypes=16, basis=False):
         """dE/d(q, sigma, epsilon) and dL/d(...) per site type of system
-        `sys` for L = sum |F - f_ref|^2; see mm_param_gradients."""
+        `sys` for L = sum |F - f_ref|^2: the adjoint kernels
+        (mm_param_gradients; 'ms' = their device time), or with basis=True
+        the basis of forward evaluations (mm_param_gradients_basis;
+        'basis_err')."""
         n = self.systems[sys].nsite
         nt = np.zeros(1, dtype=np.int32)
         st = np.zeros(n, dtype=np.int32)
@@ -262,14 +269,32 @@ class Engine:
         dE = np.zeros(3 * maxtypes)
         dL = np.zeros(3 * maxtypes)
         loss = np.zeros(1)
-        berr = np.zeros(1)
+        extra = np.zeros(1)
         fr = None if f_ref is None else np.ascontiguousarray(f_ref, dtype=np.float64).reshape(-1)
-        _check(lib().mm_param_gradients(self.h, sys, _ptr(fr), maxtypes, _ptr(nt), _ptr(st),
-                                        _ptr(tp), _ptr(dE), _ptr(dL), _ptr(loss), _ptr(berr)))
+        fn = lib().mm_param_gradients_basis if basis else lib().mm_param_gradients
+        _check(fn(self.h, sys, _ptr(fr), maxtypes, _ptr(nt), _ptr(st), _ptr(tp), _ptr(dE),
+                  _ptr(dL), _ptr(loss), _ptr(extra)))
         k = int(nt[0])
-        return dict(ntype=k, site_type=st, params=tp[:3 * k].reshape(k, 3),
-                    dE=dE[:3 * k].reshape(k, 3), dL=dL[:3 * k].reshape(k, 3),
-                    loss=float(loss[0]), basis_err=float(berr[0]))
+        out = dict(ntype=k, site_type=st, params=tp[:3 * k].reshape(k, 3),
+                   dE=dE[:3 * k].reshape(k, 3), dL=dL[:3 * k].reshape(k, 3),
+                   loss=float(loss[0]))
+        out["basis_err" if basis else "ms"] = float(extra[0])
+        return out
+
+    # site parameters and free-energy perturbation ----------------------------
+    def set_site_params(self, sys, q, sigma, epsilon):
+        """New per-site charges (internal units), sigma, epsilon of system sys."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (q, sigma, epsilon)]
+        assert all(x.size == self.systems[sys].nsite for x in a)
+        _check(lib().mm_set_site_params(self.h, sys, _ptr(a[0]), _ptr(a[1]), _ptr(a[2])))
+
+    def set_perturbation(self, sys, qB, sigmaB, epsilonB):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (qB, sigmaB, epsilonB)]
+        assert all(x.size == self.systems[sys].nsite for x in a)
+        _check(lib().mm_set_perturbation(self.h, sys, _ptr(a[0]), _ptr(a[1]), _ptr(a[2])))
+
+    def set_lambda(self, sys, lam):
+        _check(lib().mm_set_lambda(self.h, sys, float(lam)))
 
     # replica exchange --------------------------------------------------------
     def remd_setup(self, ladders, kT, seed=1):
