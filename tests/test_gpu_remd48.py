@@ -9,7 +9,8 @@ ending in an exchange attempt and a velocity redraw.
 Bars, per system:
   * forces, energies and the virial against the FP64 oracle at the start and
     at the end of the run: force relative RMS <= 1e-5, energies <= 1e-5 of
-    |U|, virial <= 1e-5 relative (Frobenius);
+    |E_elec| + |E_LJ| (a replica's two terms can nearly cancel), virial
+    <= 1e-5 relative (Frobenius);
   * the exchange decisions of both attempts equal the host restatement of
     the reference rule (mdyn.cpp:317-351) driven by the same counter-based
     random numbers and the energies the device held at the attempt;
@@ -61,7 +62,9 @@ def check_against_oracle(systems, params, e, tag):
         a, b = off[k], off[k + 1]
         of, oen, ow = O.evaluate(s, p, r=r[a:b], virial=True)
         ferr = rel(f[a:b], of)
-        eerr = max(abs(en[k, 0] - oen[0]), abs(en[k, 1] - oen[1])) / abs(oen[3])
+        # relative to the terms' size: a replica's elec and LJ terms can
+        # nearly cancel (one 216-water start sums to -1.0 from +82 and -83)
+        eerr = max(abs(en[k, 0] - oen[0]), abs(en[k, 1] - oen[1])) / (abs(oen[0]) + abs(oen[1]))
         werr = rel(w[k], ow.reshape(3, 3))
         worst = [max(worst[0], ferr), max(worst[1], eerr), max(worst[2], werr)]
         assert ferr <= 1e-5, (tag, k, ferr)
