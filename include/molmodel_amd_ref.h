@@ -113,6 +113,24 @@ void kspace_phi(struct kspace *, const cart_t *r, const double *q,
                 const tensor_t *latv, double volume, double eta, double *phi);
 int kspace_number_of_wavevectors(const struct kspace *);
 
+/* kspace_field without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same sums and accumulate (+=) semantics as kspace_field with
+ * kspace_new(cutoff, nsite), but r (AoS), q, phi and evec (AoS, may be NULL)
+ * are DEVICE pointers, latv (the reciprocal lattice, tensor_t layout) is a
+ * host pointer, and the work is queued on `stream` (a hipStream_t; NULL is
+ * the null stream) without waiting for it.  Wave vectors are summed in
+ * lexicographic order.  All other device memory the kernels use is one
+ * file-scope array of 3.9 MB per device, held until the process ends, so
+ * at most 65536 sites and 4096 wave vectors (cutoff <= 12); calls that
+ * overlap on one device share that array and race.  Returns 0; -1 with
+ * nothing queued for sizes beyond those limits or other bad arguments;
+ * -2 on a HIP error (text from mm_last_error()). */
+int kspace_field_noalloc(double cutoff, int nsite, const cart_t *r, const double *q,
+                         const tensor_t *latv, double volume, double eta, double *phi,
+                         cart_t *evec, void *stream);
+
 /* ---- pppm.h:8-64 ---- */
 typedef struct p3m *p3m_t;
 p3m_t p3m_new(int nsite, const double primitive_translation_vectors[9], double grid_spacing,
