@@ -65,9 +65,10 @@ constexpr int QCAP = 32;      // queue entries per i-group (one batch of 16 + a 
 // The pair kernel.  Each 16-lane segment owns one i-group and alternates
 // two phases over its neighbour row:
 //   filter:  16 candidates at a time; FP64 minimum image and the
-//            r^2 > rc^2 / switching decisions of rspace.c:423-445; accepted
-//            neighbours are compacted into a per-segment LDS queue with a
-//            ballot (FP32 shift, switching value and derivative);
+//            r^2 > rc^2 decision of rspace.c:423-445, the switching value
+//            and derivative (rspace.c:52-58) from the FP64 distance in FP32;
+//            accepted neighbours are compacted into a per-segment LDS queue
+//            with a ballot (FP32 shift, switching value and derivative);
 //   compute: once every segment of the wavefront holds a full batch (or has
 //            no candidates left), each lane takes one queued group pair and
 //            evaluates its site pairs in FP32, adding FP32 per-pair sums
@@ -165,7 +166,8 @@ __device__ __forceinline__ void pair_tile(
 #pragma unroll
   for (int c = 0; c < 9; c++) w[c] = 0;
 
-  const double L = P.L, invL = P.invL, rhi2 = P.rhi2, rlo2 = P.rlo2;
+  const double L = P.L, invL = P.invL, rhi2 = P.rhi2, rlo2 = P.rlo2, inv_w2 = P.inv_w2;
+  const float inv_w2f = (float)inv_w2;
   TabView T;
   T.amin = P.tabAmin; T.bmax = P.tabBmax; T.ainv = P.tabAinv; T.binv = P.tabBinv;
   T.offA = -P.tabAmin * P.tabAinv;
@@ -200,15 +202,15 @@ __device__ __forceinline__ void pair_tile(
         dz -= L * floor(dz * invL + 0.5);
         const double r2g = dx * dx + dy * dy + dz * dz;
         take = !(r2g > rhi2);
-        double sm = 1.0, dsm = 0.0;
-        if (r2g > rlo2) {                       // rspace.c:52-58
-          const double a = P.inv_w2;
-          const double z = (r2g - rlo2) * a;
-          sm = ((-6 * z + 15) * z - 10) * z * z * z + 1;
-          dsm = ((60 * z - 120) * z + 60) * z * z * a;
-        }
-        ent = make_float4((float)dx, (float)dy, (float)dz, (float)sm);
-        dsmf = (float)dsm;
+        // switching function (rspace.c:52-58) of z = (r^2 - rlo^2) / w2,
+        // branch-free in FP32 (its value and derivative are FP32 here
+        // anyway): z clamped at 0 gives S = 1, dS = 0 inside rlo
+        const float z = fmaxf((float)((r2g - rlo2) * inv_w2), 0.f);
+        const float z3 = z * z * z;
+        const float smz = fmaf(fmaf(fmaf(-6.f, z, 15.f), z, -10.f), z3, 1.f);
+        const float dsz = fmaf(fmaf(60.f, z, -120.f), z, 60.f) * z * z * inv_w2f;
+        ent = make_float4((float)dx, (float)dy, (float)dz, smz);
+        dsmf = dsz;
       }
       const unsigned long long bal = __ballot(take);
       const unsigned sb = (unsigned)(bal >> (segw * LANES)) & 0xffffu;
