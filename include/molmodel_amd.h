@@ -79,6 +79,44 @@ typedef struct {
   int geometric_sigma;       /* geometric_combining_for_sigma */
 } mm_system_desc;
 
+/* ElecLJ's defaults, for callers that fill a descriptor from ElecLJ's io
+ * parameters.
+ * mm_desc_init: zeroes *d and sets the constructor's defaults
+ *   (eleclj.cpp:13-37): smoothing_width 1, kc 5, order 3, nalias 2,
+ *   include_lj_correction 1, Ewald (kmode 0), and the values ElecLJ::init
+ *   treats as unset: rc 0, skin 0, eta -1.
+ * mm_desc_resolve: replaces the unset values as ElecLJ::init does for a
+ *   periodic box (eleclj.cpp:164-201, 229-230), with d->boxl set:
+ *   rc 0 -> 0.45 boxl (rc > 0.49 boxl is an error);
+ *   skin 0 -> 0.2 rc, or 0.499 boxl - rc when rc + 0.2 rc >= 0.5 boxl;
+ *   P3M (kmode 1) with mesh[0..2] not all positive -> the mesh of
+ *     p3m_new at p3m_grid_spacing, or at boxl / cbrt(density * boxl^3)
+ *     when p3m_grid_density > 0 (then the spacing must be the default 0.5);
+ *   eta < 0 -> Ewald: sqrt(pi kc / (rc boxl)); P3M: p3m_optimize's minimum
+ *     of the real-space and mesh RMS error estimates.
+ * Host arithmetic only: both run without a GPU. */
+void mm_desc_init(mm_system_desc *d);
+int mm_desc_resolve(mm_system_desc *d, double p3m_grid_spacing, double p3m_grid_density);
+
+/* Sites, groups and excluded pairs of ElecLJ::init_sites
+ * (eleclj.cpp:643-771) for molecules whose sites are their atoms (no M,
+ * sp2 or sp3 virtual sites).  Bonds in CSR form: the neighbours of atom a
+ * are nbr[nbr_start[a] .. nbr_start[a+1]), in the order of Atom::neighbors,
+ * each bond listed from both atoms.  Outputs:
+ *   atom_of_site[natom]  the atom at each site.  Atoms are visited in
+ *       order; a non-terminal (or isolated) atom opens a group and takes its
+ *       terminal neighbours in neighbour order; two terminal atoms bonded to
+ *       each other form one group, opened by the lower index;
+ *   group[2*natom]       [start, end) site ranges, *ngroup of them;
+ *   excl[2*maxexcl]      the 1-2 and 1-3 site pairs (i < j), sorted, each
+ *       once; *nexcl is their number (entries past maxexcl are not written);
+ *   *ncross              how many of them join two groups (the engine
+ *       requires none: mm_engine_create rejects such pairs).
+ * Host code only: runs without a GPU. */
+int mm_derive_sites(int natom, const int *nbr_start, const int *nbr, int *atom_of_site,
+                    int *group, int *ngroup, int *excl, long maxexcl, long *nexcl,
+                    long *ncross);
+
 typedef struct mm_engine mm_engine;
 
 /* energy slots returned per system by mm_get_energies */

@@ -89,6 +89,35 @@ void RSpaceEwaldPhi(int nsite, int ngroup, const int (*group)[2], const cart_t *
                     const struct PairBinary *one_four,
                     double elec_1_4_scale_factor);
 
+/* rspace.h:36-48, 74-86: non-periodic real space (clusters): every site pair
+ * of two different groups, Coulomb 1/r (screened inside rscreen_i +
+ * rscreen_j) and LJ, no cutoff; FP64 on the device.  rscreen may be NULL
+ * (no screening).  Exclusions between groups and 1-4 scaled pairs abort. */
+void RSpaceNonPeriodic(int nsite, int ngroup, const int (*group)[2], const cart_t *r,
+                       const struct PairBinary *group_excluded,
+                       const struct PairBinary *excluded,
+                       const int *site_mask, const double *q,
+                       double *phi, cart_t *evec,
+                       const double *rscreen,
+                       const double *sig, const double *eps,
+                       int geometric_combining_for_sigma,
+                       double *lj_ener, cart_t *lj_force,
+                       const struct PairBinary *one_four,
+                       double elec_1_4_scale_factor,
+                       double lj_1_4_scale_factor);
+void RSpaceNonPeriodicPhi(int nsite, int ngroup, const int (*group)[2], const cart_t *r,
+                          const struct PairBinary *group_excluded,
+                          const struct PairBinary *excluded,
+                          const int *site_mask, const double *q,
+                          double *phi, const double *rscreen,
+                          const struct PairBinary *one_four,
+                          double elec_1_4_scale_factor);
+void RSpaceNonPeriodicPhiAtGridpoints(int nsite, int ngrid, const cart_t *rsite,
+                                      const cart_t *rgrid,
+                                      const int *site_mask,
+                                      const double *q,
+                                      double *phi, const double *rscreen);
+
 /* rspace.h:64-72: potential at ngrid points from every site with a mask
  * bit (the real-space part of ElecLJ::electrostatic_potential_at_gridpoints,
  * eleclj.cpp:1295-1341; the caller adds kspace_phi over points + sites). */

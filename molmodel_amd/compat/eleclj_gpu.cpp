@@ -1,0 +1,225 @@
+/*
+ * eleclj_gpu.cpp -- ElecLJGpu (eleclj_gpu.hpp): ElecLJ's energy and forces
+ * from libmolmodel_amd.so, set up from ElecLJ's public io parameters and
+ * the MSys alone.
+ */
+#include "eleclj_gpu.hpp"
+#include "msys.hpp"
+#include "bcond.hpp"
+#include "units.h"
+#include "molmodel_amd.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+
+/* the reference's die() (util.c:11-26): message on stderr, exit status 1 */
+static void fatal(const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  fprintf(stderr, "ElecLJGpu: ");
+  vfprintf(stderr, fmt, ap);
+  fprintf(stderr, "\n");
+  va_end(ap);
+  exit(1);
+}
+
+static void check(int rc, const char *what)
+{
+  if (rc < 0) fatal("%s: %s", what, mm_last_error());
+}
+
+static bool almost_zero(double x) { return fabs(x) < small_val(); }   /* fns.c:29 */
+
+ElecLJGpu::ElecLJGpu(const ElecLJ &parameters)
+  : par(parameters), eng(0), nsite(0), elec(0), lj(0), lambda(0)
+{ }
+
+ElecLJGpu::~ElecLJGpu()
+{
+  release();
+}
+
+void ElecLJGpu::release()
+{
+  if (eng) mm_engine_destroy(eng);
+  eng = 0;
+}
+
+/* ElecLJ::types_changed (eleclj.cpp:433-502) for atom sites of one side of
+ * a perturbation: charges from fixed_charge, LJ from sigma/epsilon through
+ * general_type.  A site whose epsilon is almost zero gets no LJ. */
+void ElecLJGpu::site_params(const MSys &m, std::vector<double> &q, std::vector<double> &sig,
+                            std::vector<double> &eps, std::vector<int> &site_mask) const
+{
+  q.assign(nsite, 0.0);
+  sig.assign(nsite, 0.0);
+  eps.assign(nsite, 0.0);
+  site_mask.assign(nsite, 0);
+  for (int s = 0; s < nsite; s++) {
+    const Str &type = m.atom[atom_of_site[s]].type;
+    const double *p;
+    if ((p = par.fixed_charge.get(type)) && !almost_zero(*p)) {
+      q[s] = e_to_charge_unit(*p);
+      site_mask[s] |= MM_SITE_HAS_Q;
+    }
+    if ((p = par.screening_radius.get(type)) && *p != 0)
+      fatal("screening radius %g for type %s: screened sites are not supported", *p,
+            (const char *) type);
+    const Str *alias = par.general_type.get(type);
+    const Str &slj = alias ? *alias : type;
+    const double *ts = par.sigma.get(slj), *te = par.epsilon.get(slj);
+    if (ts && te && !almost_zero(*te)) {
+      if (*te < 0) fatal("epsilon must be >= 0 for type %s", (const char *) slj);
+      if (*ts <= 0) fatal("sigma must be > 0 for type %s", (const char *) slj);
+      site_mask[s] |= MM_SITE_HAS_LJ;
+      sig[s] = *ts;
+      eps[s] = *te;
+    }
+  }
+}
+
+void ElecLJGpu::init(MSys *m)
+{
+  Potential::init(m);
+  release();
+  const BoundaryConditions &bc = *m->boundary_conditions;
+  if (bc.type != cubic) fatal("only cubic periodic boxes run on the device");
+  if (par.msite.size() || par.sp3.size() || par.sp2.size())
+    fatal("virtual (M, sp2, sp3) sites are not supported");
+  if (par.bond_charge_increment.size()) fatal("bond charge increments are not supported");
+  if (par.one_three_interaction.size()) fatal("1-3 interactions are not supported");
+  if (!almost_zero(par.elec_1_4_scale_factor - 1) || !almost_zero(par.lj_1_4_scale_factor - 1))
+    fatal("1-4 scale factors other than 1 are not supported");
+  if (!par.include_rspace || !par.include_kspace || !par.include_self)
+    fatal("the real-space, k-space and self terms are always evaluated");
+  if (par.use_p3m && par.p3m_ik_differentiate) fatal("ik differentiation is not supported");
+
+  /* eleclj.cpp:643-771: sites, groups, 1-2 and 1-3 exclusions */
+  const int natom = m->atom.size();
+  std::vector<int> start(natom + 1, 0), nbr;
+  for (int a = 0; a < natom; a++) {
+    LstItr<int> j;
+    for (j.init(m->atom[a].neighbors); j.ok(); j.next()) nbr.push_back(j());
+    start[a + 1] = (int) nbr.size();
+  }
+  nsite = natom;
+  atom_of_site.assign(natom, 0);
+  std::vector<int> group(2 * (size_t) natom);
+  int ngroup = 0;
+  long nexcl = 0, ncross = 0;
+  check(mm_derive_sites(natom, start.data(), nbr.data(), atom_of_site.data(), group.data(),
+                        &ngroup, 0, 0, &nexcl, &ncross), "mm_derive_sites");
+  std::vector<int> excl(2 * (size_t) nexcl + 2);
+  check(mm_derive_sites(natom, start.data(), nbr.data(), atom_of_site.data(), group.data(),
+                        &ngroup, excl.data(), nexcl, &nexcl, &ncross), "mm_derive_sites");
+  if (ncross) fatal("%ld excluded pairs join two groups: not supported on the device", ncross);
+
+  std::vector<double> q, sig, eps;
+  site_params(*m, q, sig, eps, mask);
+
+  /* eleclj.cpp:164-265: cutoffs, width and Ewald or P3M parameters */
+  mm_system_desc d;
+  mm_desc_init(&d);
+  d.nsite = nsite;
+  d.ngroup = ngroup;
+  d.group = group.data();
+  d.q = q.data();
+  d.sigma = sig.data();
+  d.epsilon = eps.data();
+  d.site_mask = mask.data();
+  d.nexcl = (int) nexcl;
+  d.excl = excl.data();
+  d.boxl = bc.min_diameter();
+  d.rc = par.rspace_cutoff;
+  d.smoothing_width = par.smoothing_width;
+  d.skin = par.neighbor_list_width;
+  d.eta = par.ewald_screening;
+  d.kmode = par.use_p3m ? 1 : 0;
+  d.kc = par.kspace_cutoff;
+  d.order = par.p3m_assignment_order;
+  d.nalias = par.p3m_alias_sum_limit;
+  d.include_lj_correction = par.include_lj_correction ? 1 : 0;
+  d.geometric_sigma = par.geometric_combining_for_sigma ? 1 : 0;
+  check(mm_desc_resolve(&d, par.p3m_grid_spacing, par.p3m_grid_density), "mm_desc_resolve");
+  check(mm_engine_create(1, &d, &eng), "mm_engine_create");
+  pos.assign(3 * (size_t) nsite, 0.0);
+  force.assign(3 * (size_t) nsite, 0.0);
+  lambda = 0;
+}
+
+void ElecLJGpu::add_to_energy_and_forces(double &u, Cartesian *f)
+{
+  if (!eng) fatal("add_to_energy_and_forces before init");
+  for (int s = 0; s < nsite; s++) {
+    const Cartesian &r = msys->atom[atom_of_site[s]].position;
+    pos[3 * s] = r.x;
+    pos[3 * s + 1] = r.y;
+    pos[3 * s + 2] = r.z;
+  }
+  check(mm_set_positions(eng, pos.data()), "mm_set_positions");
+  check(mm_compute(eng, 0), "mm_compute");
+  double en[MM_NE];
+  check(mm_get_energies(eng, en), "mm_get_energies");
+  check(mm_get_forces(eng, force.data()), "mm_get_forces");
+  elec = en[MM_E_ELEC];
+  lj = en[MM_E_LJ];
+  u += en[MM_E_TOTAL];
+  for (int s = 0; s < nsite; s++) {
+    Cartesian &fa = f[atom_of_site[s]];
+    fa.x += force[3 * s];
+    fa.y += force[3 * s + 1];
+    fa.z += force[3 * s + 2];
+  }
+}
+
+void ElecLJGpu::write() const
+{
+  printf("Electrostatic energy: %.10g kcal/mol\nLJ energy: %.10g kcal/mol\n", elec, lj);
+  fflush(stdout);
+}
+
+void ElecLJGpu::scale(double s)
+{
+  if (eng) check(mm_scale_system(eng, 0, s), "mm_scale_system");
+}
+
+/* ElecLJ::types_changed() = types_changed(*msys) then set_lambda(0)
+ * (eleclj.cpp:267-271) */
+void ElecLJGpu::types_changed()
+{
+  if (!eng) return;
+  std::vector<double> q, sig, eps;
+  std::vector<int> m2;
+  site_params(*msys, q, sig, eps, m2);
+  if (m2 != mask) init(msys);   /* a site gained or lost a charge or LJ: new engine */
+  else check(mm_set_site_params(eng, 0, q.data(), sig.data(), eps.data()), "mm_set_site_params");
+  lambda = 0;
+}
+
+void ElecLJGpu::types_changed(const MSys &perturbed)
+{
+  if (!eng) fatal("types_changed before init");
+  if (perturbed.atom.size() != msys->atom.size())
+    fatal("the perturbed system has a different number of atoms");
+  std::vector<double> qa, sa, ea, qb, sb, eb;
+  std::vector<int> ma, mb;
+  site_params(*msys, qa, sa, ea, ma);
+  site_params(perturbed, qb, sb, eb, mb);
+  for (int s = 0; s < nsite; s++)
+    if ((ma[s] | mb[s]) != mask[s])
+      fatal("site %d gains a charge or LJ term under the perturbation: not supported", s);
+  check(mm_set_site_params(eng, 0, qa.data(), sa.data(), ea.data()), "mm_set_site_params");
+  check(mm_set_perturbation(eng, 0, qb.data(), sb.data(), eb.data()), "mm_set_perturbation");
+  check(mm_set_lambda(eng, 0, 0.0), "mm_set_lambda");
+  lambda = 0;
+}
+
+void ElecLJGpu::set_lambda(double l)
+{
+  if (!eng) fatal("set_lambda before init");
+  check(mm_set_lambda(eng, 0, l), "mm_set_lambda");
+  lambda = l;
+}

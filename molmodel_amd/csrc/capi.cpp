@@ -1,6 +1,8 @@
 // capi.cpp -- the extern "C" engine API declared in include/molmodel_amd.h.
 #include "../../include/molmodel_amd.h"
 #include "handle.hpp"
+#include <cfloat>
+#include <cmath>
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -88,6 +90,79 @@ int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out) {
     return rc;
   }
   *out = m;
+  return MM_OK;
+}
+
+// ElecLJ::ElecLJ (eleclj.cpp:13-37): the io parameters' defaults; the
+// "unset" values are the ones ElecLJ::init replaces (rc 0, skin 0, eta -1)
+void mm_desc_init(mm_system_desc *d) {
+  if (!d) return;
+  memset(d, 0, sizeof *d);
+  d->rc = 0;
+  d->smoothing_width = 1;
+  d->skin = 0;
+  d->kc = 5;
+  d->eta = -1;
+  d->kmode = 0;
+  d->order = 3;
+  d->nalias = 2;
+  d->ik = 0;
+  d->include_lj_correction = 1;
+  d->geometric_sigma = 0;
+}
+
+// ElecLJ::init (eleclj.cpp:164-201, 229-230) for a cubic box
+int mm_desc_resolve(mm_system_desc *d, double p3m_grid_spacing, double p3m_grid_density) {
+  if (!d || !(d->boxl > 0)) {
+    set_last_error("mm_desc_resolve: a positive box length is needed");
+    return MM_ERR_ARG;
+  }
+  const double L = d->boxl, small = 64 * DBL_EPSILON;   // fns.c:22-29
+  if (fabs(d->rc) < small) d->rc = 0.45 * L;
+  if (d->rc > 0.49 * L) {
+    set_last_error("mm_desc_resolve: rspace_cutoff must be <= 0.49 * minimum diameter of box "
+                   "(eleclj.cpp:168-170)");
+    return MM_ERR_ARG;
+  }
+  if (!(d->rc > 0)) {
+    set_last_error("mm_desc_resolve: rspace_cutoff must be positive");
+    return MM_ERR_ARG;
+  }
+  if (fabs(d->skin) < small) {
+    d->skin = 0.2 * d->rc;
+    if (d->rc + d->skin >= 0.5 * L) d->skin = 0.499 * L - d->rc;
+  }
+  if (d->smoothing_width >= d->rc) {
+    set_last_error("mm_desc_resolve: smoothing_width must be less than the real-space cutoff "
+                   "(eleclj.cpp:180-182)");
+    return MM_ERR_ARG;
+  }
+  if (d->kmode == 1) {
+    double h = p3m_grid_spacing;
+    if (p3m_grid_density > 0) {
+      if (fabs(h - 0.5) >= small) {
+        set_last_error("mm_desc_resolve: only one of p3m_grid_spacing and p3m_grid_density "
+                       "(eleclj.cpp:193-195)");
+        return MM_ERR_ARG;
+      }
+      h = L / cbrt(p3m_grid_density * L * L * L);
+    }
+    if (!(h > 0)) {
+      set_last_error("mm_desc_resolve: grid spacing must be positive (pppm.c:412)");
+      return MM_ERR_ARG;
+    }
+    if (d->mesh[0] <= 0 || d->mesh[1] <= 0 || d->mesh[2] <= 0) {
+      const int n = multiple_of_2357_near(floor(L / h + 0.5));   // pppm.c:424-431
+      d->mesh[0] = d->mesh[1] = d->mesh[2] = n;
+    }
+    if (d->eta < 0) {
+      const double latv[9] = {L, 0, 0, 0, L, 0, 0, 0, L};
+      double cr = 0, ck = 0;
+      p3m_optimize_eta(latv, d->mesh, d->nalias, d->order, h, d->rc, &d->eta, &cr, &ck);
+    }
+  } else if (d->eta < 0) {
+    d->eta = sqrt(M_PI * d->kc / (d->rc * L));   // eleclj.cpp:229-230
+  }
   return MM_OK;
 }
 

@@ -175,6 +175,12 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in);
 void tensor_inverse(double *a, const double *b);
 double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl,
                 double inv_4eta2, double inv_v, int nalias, int order, double *chi_acc);
+int multiple_of_2357_near(double x);
+double p3m_influence(const double latv[9], const int ngrid[3], int nalias, int order, double eta,
+                     std::vector<double> *green);
+void p3m_optimize_eta(const double latv[9], const int ngrid[3], int nalias, int order,
+                      double grid_spacing, double rc, double *eta, double *chi_rspace,
+                      double *chi_p3m);
 void cheb_fit(double a, double b, double (*f)(double), double *c4);
 void engine_destroy(Engine &e);
 void lj_tail_factors(const SystemInput &s, double *ca, double *cb);
@@ -205,6 +211,13 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies = 1);
 inline int launch_kspace(Engine &e, int want_virial) { return launch_kspace(e, want_virial, e.stream); }
 inline int launch_p3m(Engine &e, int want_virial) { return launch_p3m(e, want_virial, e.stream); }
 int launch_finish(Engine &e, int mode);
+// non-periodic real space (rspace.hip): mode 0 RSpaceNonPeriodic, 1 its
+// phi-only variant; out 7 per site, upart one LJ energy per 256 sites
+int launch_nonperiodic(int mode, const double4 *site, const int4 *info, const double2 *lj,
+                       const double *rscreen, int n, int geometric, double *out, double *upart,
+                       hipStream_t st);
+int launch_nonperiodic_points(const double4 *site, const double *rscreen, int n,
+                              const double *pts, int npts, double *phi, hipStream_t st);
 // k-space structure factors and field of one system in buffers the caller
 // provides (kspace.hip)
 int launch_kspace_raw(const SysParams *sp, const double4 *kvec, int nk, int nsite,

@@ -385,6 +385,153 @@ extern "C" void RSpaceEwaldPhiAtGridpoints(int nsite, int ngrid, const cart_t *r
   for (int j = 0; j < ngrid; j++) phi[j] += out[j];
 }
 
+// ------------------------------------------------------------ non-periodic
+// RSpaceNonPeriodic / RSpaceNonPeriodicPhi / RSpaceNonPeriodicPhiAtGridpoints
+// (rspace.c:560-640, 784-900) on the device in FP64: all pairs of sites in
+// different groups, no cutoff.  Exclusions between groups and 1-4 scaled
+// pairs abort as in the periodic routines (pairs inside a group are never
+// evaluated here, as in the reference).
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  explicit DevBuf(size_t bytes) { HIPCHK(hipMalloc(&p, bytes ? bytes : 8)); }
+  ~DevBuf() { hipFree(p); }
+  template <class T> T *as() { return static_cast<T *>(p); }
+};
+
+void nonperiodic_checks(const char *who, int nsite, int ngroup,
+                        const struct PairBinary *group_excluded,
+                        const struct PairBinary *one_four) {
+  if (nsite < 0 || ngroup < 0) die_msg(who, "nsite and ngroup must be >= 0");
+  if (group_excluded)
+    for (int g = 0; g < ngroup; g++)
+      if (PairBinary_number_of_elements(group_excluded)[g])
+        die_msg(who, "exclusions between groups are not supported on the device");
+  if (one_four)
+    for (int i = 0; i < nsite; i++)
+      if (PairBinary_number_of_elements(one_four)[i])
+        die_msg(who, "1-4 scaled pairs are not supported on the device");
+}
+
+// 7 doubles per site (phi, evec, lj_force) and the LJ energy
+void nonperiodic_on_device(const char *who, int mode, int nsite, int ngroup,
+                           const int (*group)[2], const cart_t *r, const int *site_mask,
+                           const double *q, const double *rscreen, const double *sig,
+                           const double *eps, int geometric, std::vector<double> &out,
+                           double &ulj) {
+  std::vector<double4> site(nsite);
+  std::vector<int4> info(nsite, make_int4(-1, 0, 0, 0));
+  std::vector<double2> lj(nsite, make_double2(0, 0));
+  std::vector<double> rs(nsite, 0.0);
+  for (int g = 0; g < ngroup; g++)
+    for (int i = group[g][0]; i < group[g][1]; i++) {
+      if (i < 0 || i >= nsite) die_msg(who, "group out of range");
+      info[i] = make_int4(g, site_mask[i], i == group[g][0] ? 1 : 0, 0);
+    }
+  for (int i = 0; i < nsite; i++) {
+    if (info[i].x < 0) die_msg(who, "site not in any group");
+    site[i] = make_double4(r[i].x, r[i].y, r[i].z, q[i]);
+    if (sig && eps) lj[i] = make_double2(sig[i], eps[i]);
+    if (rscreen) rs[i] = rscreen[i];
+  }
+  const int nblk = (nsite + 255) / 256;
+  DevBuf ds(sizeof(double4) * nsite), di(sizeof(int4) * nsite), dl(sizeof(double2) * nsite),
+      dr(sizeof(double) * nsite), dout(sizeof(double) * 7 * nsite), du(sizeof(double) * nblk);
+  HIPCHK(hipMemcpy(ds.p, site.data(), sizeof(double4) * nsite, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(di.p, info.data(), sizeof(int4) * nsite, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dl.p, lj.data(), sizeof(double2) * nsite, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dr.p, rs.data(), sizeof(double) * nsite, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(dout.p, 0, sizeof(double) * 7 * nsite));
+  if (launch_nonperiodic(mode, ds.as<double4>(), di.as<int4>(), dl.as<double2>(), dr.as<double>(),
+                         nsite, geometric, dout.as<double>(), du.as<double>(), nullptr))
+    die_msg(who, last_error());
+  HIPCHK(hipDeviceSynchronize());
+  out.resize(7 * (size_t)nsite);
+  HIPCHK(hipMemcpy(out.data(), dout.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost));
+  ulj = 0;
+  if (mode == 0) {
+    std::vector<double> up(nblk);
+    HIPCHK(hipMemcpy(up.data(), du.p, sizeof(double) * nblk, hipMemcpyDeviceToHost));
+    for (double x : up) ulj += x;
+  }
+}
+
+}  // namespace
+
+extern "C" void RSpaceNonPeriodic(int nsite, int ngroup, const int (*group)[2], const cart_t *r,
+                                  const struct PairBinary *group_excluded,
+                                  const struct PairBinary *excluded, const int *site_mask,
+                                  const double *q, double *phi, cart_t *evec,
+                                  const double *rscreen, const double *sig, const double *eps,
+                                  int geometric_combining_for_sigma, double *lj_ener,
+                                  cart_t *lj_force, const struct PairBinary *one_four,
+                                  double elec_1_4_scale_factor, double lj_1_4_scale_factor) {
+  (void)excluded;   // consulted only for group-excluded pairs, which abort here
+  (void)elec_1_4_scale_factor;
+  (void)lj_1_4_scale_factor;
+  const char *who = "RSpaceNonPeriodic";
+  nonperiodic_checks(who, nsite, ngroup, group_excluded, one_four);
+  if (nsite == 0) return;
+  std::vector<double> out;
+  double u = 0;
+  nonperiodic_on_device(who, 0, nsite, ngroup, group, r, site_mask, q, rscreen, sig, eps,
+                        geometric_combining_for_sigma, out, u);
+  for (int i = 0; i < nsite; i++) {
+    const double *o = &out[7 * (size_t)i];
+    phi[i] += o[0];
+    evec[i].x += o[1]; evec[i].y += o[2]; evec[i].z += o[3];
+    lj_force[i].x += o[4]; lj_force[i].y += o[5]; lj_force[i].z += o[6];
+  }
+  *lj_ener += u;
+}
+
+extern "C" void RSpaceNonPeriodicPhi(int nsite, int ngroup, const int (*group)[2], const cart_t *r,
+                                     const struct PairBinary *group_excluded,
+                                     const struct PairBinary *excluded, const int *site_mask,
+                                     const double *q, double *phi, const double *rscreen,
+                                     const struct PairBinary *one_four,
+                                     double elec_1_4_scale_factor) {
+  (void)excluded;
+  (void)elec_1_4_scale_factor;
+  const char *who = "RSpaceNonPeriodicPhi";
+  nonperiodic_checks(who, nsite, ngroup, group_excluded, one_four);
+  if (nsite == 0) return;
+  std::vector<double> out;
+  double u = 0;
+  nonperiodic_on_device(who, 1, nsite, ngroup, group, r, site_mask, q, rscreen, nullptr, nullptr,
+                        0, out, u);
+  for (int i = 0; i < nsite; i++) phi[i] += out[7 * (size_t)i];
+}
+
+extern "C" void RSpaceNonPeriodicPhiAtGridpoints(int nsite, int ngrid, const cart_t *rsite,
+                                                 const cart_t *rgrid, const int *site_mask,
+                                                 const double *q, double *phi,
+                                                 const double *rscreen) {
+  const char *who = "RSpaceNonPeriodicPhiAtGridpoints";
+  if (nsite < 0 || ngrid < 0) die_msg(who, "nsite and ngrid must be >= 0");
+  if (nsite == 0 || ngrid == 0) return;
+  std::vector<double4> site(nsite);
+  std::vector<double> rs(nsite, 0.0);
+  for (int i = 0; i < nsite; i++) {
+    site[i] = make_double4(rsite[i].x, rsite[i].y, rsite[i].z, site_mask[i] ? q[i] : 0.0);
+    if (rscreen) rs[i] = rscreen[i];
+  }
+  DevBuf ds(sizeof(double4) * nsite), dr(sizeof(double) * nsite), dp(sizeof(double) * 3 * ngrid),
+      dphi(sizeof(double) * ngrid);
+  HIPCHK(hipMemcpy(ds.p, site.data(), sizeof(double4) * nsite, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dr.p, rs.data(), sizeof(double) * nsite, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dp.p, rgrid, sizeof(double) * 3 * (size_t)ngrid, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(dphi.p, 0, sizeof(double) * ngrid));
+  if (launch_nonperiodic_points(ds.as<double4>(), dr.as<double>(), nsite, dp.as<double>(), ngrid,
+                                dphi.as<double>(), nullptr))
+    die_msg(who, last_error());
+  HIPCHK(hipDeviceSynchronize());
+  std::vector<double> out(ngrid);
+  HIPCHK(hipMemcpy(out.data(), dphi.p, sizeof(double) * ngrid, hipMemcpyDeviceToHost));
+  for (int j = 0; j < ngrid; j++) phi[j] += out[j];
+}
+
 // ------------------------------------------------------------ k space
 struct kspace {
   double kc;
@@ -516,7 +663,7 @@ struct p3m {
   int have_green = 0;
 };
 
-static int multiple_of_2357_near(double x) {   // pppm.c:277-292
+int mmamd::multiple_of_2357_near(double x) {   // pppm.c:277-292
   int best = 1;
   double bestdiff = fabs(x - 1);
   for (int p2 = 1; p2 <= x; p2 *= 2)
@@ -629,28 +776,37 @@ extern "C" p3m_t p3m_new(int nsite, const double ptv[9], double grid_spacing, in
   return p;
 }
 
-extern "C" void p3m_init(p3m_t p, double eta, double *chi) {   // pppm.c:313-393
-  const int nx = p->ngrid[0], ny = p->ngrid[1], nz = p->ngrid[2], nzh = nz / 2 + 1;
+// pppm.c:313-393: the influence function of an ngrid mesh over the cell
+// latv (columns are lattice vectors) at screening eta, into *green when
+// given (half spectrum [ix][iy][iz]), and the RMS force error estimate.
+double mmamd::p3m_influence(const double latv[9], const int ngrid[3], int nalias, int order, double eta,
+                     std::vector<double> *green) {
+  const int nx = ngrid[0], ny = ngrid[1], nz = ngrid[2], nzh = nz / 2 + 1;
   double inv[9], rl[9];
-  tensor_inverse(inv, p->latv);
+  tensor_inverse(inv, latv);
   for (int a = 0; a < 3; a++)
     for (int b = 0; b < 3; b++) rl[3 * a + b] = 2 * M_PI * inv[3 * b + a];
-  const double *L = p->latv;
+  const double *L = latv;
   const double det = -L[6] * L[4] * L[2] + L[3] * L[7] * L[2] + L[6] * L[1] * L[5] -
                      L[0] * L[7] * L[5] - L[3] * L[1] * L[8] + L[0] * L[4] * L[8];
   const double inv_v = 1.0 / fabs(det), inv_4eta2 = 1 / ((2 * eta) * (2 * eta));
-  p->green.assign((size_t)nx * ny * nzh, 0.0);
+  if (green) green->assign((size_t)nx * ny * nzh, 0.0);
   double c = 0;
   for (int ix = 0; ix < nx; ix++) {
     const int jx = ix > nx / 2 ? ix - nx : ix;
     for (int iy = 0; iy < ny; iy++) {
       const int jy = iy > ny / 2 ? iy - ny : iy;
-      for (int iz = 0; iz < nzh; iz++)
-        p->green[((size_t)ix * ny + iy) * nzh + iz] =
-            green_at(jx, jy, iz, nx, ny, nz, rl, inv_4eta2, inv_v, p->nalias, p->order, &c);
+      for (int iz = 0; iz < nzh; iz++) {
+        const double g = green_at(jx, jy, iz, nx, ny, nz, rl, inv_4eta2, inv_v, nalias, order, &c);
+        if (green) (*green)[((size_t)ix * ny + iy) * nzh + iz] = g;
+      }
     }
   }
-  *chi = pow(inv_v, 1.0 / 3.0) * sqrt(2 * fabs(c));
+  return pow(inv_v, 1.0 / 3.0) * sqrt(2 * fabs(c));
+}
+
+extern "C" void p3m_init(p3m_t p, double eta, double *chi) {   // pppm.c:313-393
+  *chi = p3m_influence(p->latv, p->ngrid, p->nalias, p->order, eta, &p->green);
   p->eta = eta;
   p->have_green = 1;
   p3m_upload_params(p);
@@ -719,19 +875,22 @@ extern "C" void p3m_delete(p3m_t p) {
   delete p;
 }
 
-extern "C" void p3m_optimize(p3m_t p, double rc, double *eta, double *chi_rspace, double *chi_p3m) {
-  const double *L = p->latv;
+// pppm.c:667-716: minimise chi_rspace^2 + chi_p3m^2 over eta
+void mmamd::p3m_optimize_eta(const double latv[9], const int ngrid[3], int nalias, int order,
+                      double grid_spacing, double rc, double *eta, double *chi_rspace,
+                      double *chi_p3m) {
+  const double *L = latv;
   const double v = fabs(-L[6] * L[4] * L[2] + L[3] * L[7] * L[2] + L[6] * L[1] * L[5] -
                         L[0] * L[7] * L[5] - L[3] * L[1] * L[8] + L[0] * L[4] * L[8]);
   double cr = 0, ck = 0;
   auto f = [&](double x) {   // pppm.c:672-680
     if (x < 0) return 1e8;
     cr = 2 / sqrt(rc / pow(v, 1.0 / 3.0)) * exp(-(rc * x) * (rc * x));
-    p3m_init(p, x, &ck);
+    ck = p3m_influence(latv, ngrid, nalias, order, x, nullptr);
     return cr * cr + ck * ck;
   };
   // one-dimensional Nelder-Mead: start 0.1/h, step 0.1, size tolerance 1e-4
-  double x0 = 0.1 / p->grid_spacing, x1 = x0 + 0.1;
+  double x0 = 0.1 / grid_spacing, x1 = x0 + 0.1;
   double f0 = f(x0), f1 = f(x1);
   for (int it = 1; it <= 100; it++) {
     if (f1 < f0) { std::swap(x0, x1); std::swap(f0, f1); }
@@ -748,8 +907,14 @@ extern "C" void p3m_optimize(p3m_t p, double rc, double *eta, double *chi_rspace
     }
   }
   if (f1 < f0) { std::swap(x0, x1); std::swap(f0, f1); }
-  f(x0);   // leave the influence function and both estimates at the returned eta
+  f(x0);   // both estimates at the returned eta
   *eta = x0;
   *chi_rspace = cr;
   *chi_p3m = ck;
+}
+
+extern "C" void p3m_optimize(p3m_t p, double rc, double *eta, double *chi_rspace, double *chi_p3m) {
+  p3m_optimize_eta(p->latv, p->ngrid, p->nalias, p->order, p->grid_spacing, rc, eta, chi_rspace,
+                   chi_p3m);
+  p3m_init(p, *eta, chi_p3m);   // leave the influence function at the returned eta
 }

@@ -584,6 +584,177 @@ int launch_phi_points(Engine &e, const double4 *sites, int nsite, const double *
   return 0;
 }
 
+// ---- non-periodic real space (rspace.c:560-640, 784-900), FP64 ----------
+// Every site pair of two different groups, no cutoff, no minimum image:
+// Coulomb 1/r, or inside the screening radius rs = rscreen_i + rscreen_j
+// the screened form b0 = 3/(2 rs) - r^2/(2 rs^3), b1 = 1/rs^3
+// (rspace.c:61-64); Lennard-Jones with Lorentz-Berthelot or geometric sigma
+// (rspace.c:258-297).  One thread per site (or grid point), the other
+// sites streamed through LDS; each thread sums its own site's terms, so
+// the two members of a pair are visited from both sides.
+//   info: per site (group, mask, 1 if first site of its group, 0)
+//   lj:   per site (sigma, epsilon)
+//   MODE 0 (RSpaceNonPeriodic): out 7 per site (phi, evec, lj_force) and
+//          per block the LJ energy of the block's sites (half of each pair);
+//   MODE 1 (RSpaceNonPeriodicPhi): phi only, with that routine's pair rule:
+//          the two first sites always, other pairs when their masks share
+//          any bit (rspace.c:801-822).
+constexpr int NP_TB = 256;
+
+__device__ __forceinline__ void np_coulomb(double r2, double rs, double &b0, double &b1) {
+  if (r2 < rs * rs) {   // screening_function, rspace.c:61-64
+    b1 = 1.0 / (rs * rs * rs);
+    b0 = -0.5 * r2 * b1 + 1.5 / rs;
+  } else {
+    const double invr = 1.0 / sqrt(r2);
+    b0 = invr;
+    b1 = invr * invr * invr;
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(NP_TB) k_nonperiodic(const double4 *__restrict__ site,
+                                                      const int4 *__restrict__ info,
+                                                      const double2 *__restrict__ lj,
+                                                      const double *__restrict__ rscreen, int n,
+                                                      int geometric, double *__restrict__ out,
+                                                      double *__restrict__ upart) {
+  __shared__ double4 ss[NP_TB];
+  __shared__ int4 si[NP_TB];
+  __shared__ double2 sl[NP_TB];
+  __shared__ double sr[NP_TB];
+  __shared__ double red[NP_TB / 64];
+  const int i = blockIdx.x * NP_TB + threadIdx.x;
+  const bool on = i < n;
+  const double4 a = on ? site[i] : make_double4(0, 0, 0, 0);
+  const int4 ia = on ? info[i] : make_int4(-1, 0, 0, 0);
+  const double2 la = on ? lj[i] : make_double2(0, 0);
+  const double ra = on ? rscreen[i] : 0.0;
+  double phi = 0, ex = 0, ey = 0, ez = 0, fx = 0, fy = 0, fz = 0, u = 0;
+  for (int t0 = 0; t0 < n; t0 += NP_TB) {
+    __syncthreads();
+    const int t = t0 + threadIdx.x;
+    if (t < n) {
+      ss[threadIdx.x] = site[t];
+      si[threadIdx.x] = info[t];
+      sl[threadIdx.x] = lj[t];
+      sr[threadIdx.x] = rscreen[t];
+    }
+    __syncthreads();
+    const int m = min(NP_TB, n - t0);
+    if (!on) continue;
+    for (int k = 0; k < m; k++) {
+      const int4 ib = si[k];
+      if (ib.x == ia.x) continue;   // same group
+      const double4 b = ss[k];
+      const double x = a.x - b.x, y = a.y - b.y, z = a.z - b.z;
+      const double r2 = x * x + y * y + z * z;
+      const int mask = ia.y & ib.y;
+      if (MODE == 1) {
+        if (!((ia.z && ib.z) || mask)) continue;
+        double b0, b1;
+        np_coulomb(r2, ra + sr[k], b0, b1);
+        phi += b.w * b0;
+        continue;
+      }
+      if (mask & SITE_HAS_Q) {
+        double b0, b1;
+        np_coulomb(r2, ra + sr[k], b0, b1);
+        phi += b.w * b0;
+        ex += b.w * b1 * x;
+        ey += b.w * b1 * y;
+        ez += b.w * b1 * z;
+      }
+      if (mask & SITE_HAS_LJ) {
+        const double2 lb = sl[k];
+        const double s2 = geometric ? la.x * lb.x : 0.25 * (la.x + lb.x) * (la.x + lb.x);
+        const double s6 = s2 * s2 * s2;
+        const double bb = 4 * sqrt(la.y * lb.y) * s6, aa = bb * s6;
+        const double ir2 = 1.0 / r2, ir6 = ir2 * ir2 * ir2;
+        const double du = (12 * aa * ir6 * ir6 - 6 * bb * ir6) * ir2;
+        u += 0.5 * (aa * ir6 * ir6 - bb * ir6);
+        fx += du * x;
+        fy += du * y;
+        fz += du * z;
+      }
+    }
+  }
+  if (on) {
+    double *o = out + 7L * i;
+    o[0] = phi;
+    if (MODE == 0) {
+      o[1] = ex; o[2] = ey; o[3] = ez;
+      o[4] = fx; o[5] = fy; o[6] = fz;
+    }
+  }
+  if (MODE == 0) {   // the block's LJ energy, reduced in a fixed order
+    for (int o = 32; o >= 1; o >>= 1) u += __shfl_xor(u, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0;
+      for (int w = 0; w < NP_TB / 64; w++) t += red[w];
+      upart[blockIdx.x] = t;
+    }
+  }
+}
+
+// RSpaceNonPeriodicPhiAtGridpoints (rspace.c:858-890): potential at each
+// point from every site with a mask bit, screened within rscreen_i
+__global__ void __launch_bounds__(NP_TB) k_nonperiodic_points(const double4 *__restrict__ site,
+                                                             const double *__restrict__ rscreen,
+                                                             int n, const double *__restrict__ pts,
+                                                             int npts, double *__restrict__ phi) {
+  __shared__ double4 ss[NP_TB];
+  __shared__ double sr[NP_TB];
+  const int j = blockIdx.x * NP_TB + threadIdx.x;
+  const bool on = j < npts;
+  const double gx = on ? pts[3 * j] : 0, gy = on ? pts[3 * j + 1] : 0, gz = on ? pts[3 * j + 2] : 0;
+  double acc = 0;
+  for (int t0 = 0; t0 < n; t0 += NP_TB) {
+    __syncthreads();
+    if (t0 + (int)threadIdx.x < n) {
+      ss[threadIdx.x] = site[t0 + threadIdx.x];
+      sr[threadIdx.x] = rscreen[t0 + threadIdx.x];
+    }
+    __syncthreads();
+    const int m = min(NP_TB, n - t0);
+    if (!on) continue;
+    for (int k = 0; k < m; k++) {
+      const double4 s = ss[k];   // w = 0 for sites without any mask bit
+      const double x = s.x - gx, y = s.y - gy, z = s.z - gz;
+      double b0, b1;
+      np_coulomb(x * x + y * y + z * z, sr[k], b0, b1);
+      acc += s.w * b0;
+    }
+  }
+  if (on) phi[j] += acc;
+}
+
+int launch_nonperiodic(int mode, const double4 *site, const int4 *info, const double2 *lj,
+                       const double *rscreen, int n, int geometric, double *out, double *upart,
+                       hipStream_t st) {
+  if (n <= 0) return 0;
+  const dim3 grid((n + NP_TB - 1) / NP_TB);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_nonperiodic<0>, grid, dim3(NP_TB), 0, st, site, info, lj, rscreen, n,
+                       geometric, out, upart);
+  else
+    hipLaunchKernelGGL(k_nonperiodic<1>, grid, dim3(NP_TB), 0, st, site, info, lj, rscreen, n,
+                       geometric, out, upart);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_nonperiodic_points(const double4 *site, const double *rscreen, int n,
+                              const double *pts, int npts, double *phi, hipStream_t st) {
+  if (n <= 0 || npts <= 0) return 0;
+  hipLaunchKernelGGL(k_nonperiodic_points, dim3((npts + NP_TB - 1) / NP_TB), dim3(NP_TB), 0, st,
+                     site, rscreen, n, pts, npts, phi);
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
 int pair_tiles(int ngroup) { return (ngroup + GPB - 1) / GPB; }
 
 int launch_pair(Engine &e, int field_mode, int want_virial) {

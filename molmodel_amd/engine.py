@@ -76,6 +76,10 @@ def lib():
         L.mm_scale_system.argtypes = [_P, _I, _D]
         L.mm_nl_builds.restype = C.c_longlong
         L.mm_nl_builds.argtypes = [_P]
+        L.mm_desc_init.restype = None
+        L.mm_desc_init.argtypes = [_P]
+        L.mm_desc_resolve.argtypes = [_P, _D, _D]
+        L.mm_derive_sites.argtypes = [_I, _P, _P, _P, _P, _P, _P, _L, _P, _P]
         _lib = L
     return _lib
 
@@ -108,6 +112,52 @@ def tune_kspace(boxl, nsite, rc, tol=1e-4, allow_ewald=True):
         out["mesh"] = tuple(ch.mesh)
         out["order"] = ch.order
     return out
+
+
+def eleclj_defaults(boxl, p3m=False, p3m_grid_spacing=0.5, p3m_grid_density=-1.0, **given):
+    """ElecLJ's parameter defaults for a cubic box (mm_desc_init, then
+    mm_desc_resolve): pass any of rc, skin, smoothing_width, eta, kc, order,
+    nalias, mesh to fix them; returns the resolved parameters as a dict of
+    the waterbox params keys.  Host only."""
+    d = SystemDesc()
+    lib().mm_desc_init(C.byref(d))
+    d.boxl = float(boxl)
+    d.kmode = 1 if p3m else 0
+    for k in ("rc", "skin", "smoothing_width", "eta", "kc"):
+        if k in given:
+            setattr(d, k, float(given[k]))
+    for k in ("order", "nalias"):
+        if k in given:
+            setattr(d, k, int(given[k]))
+    if "mesh" in given:
+        d.mesh = (_I * 3)(*given["mesh"])
+    _check(lib().mm_desc_resolve(C.byref(d), float(p3m_grid_spacing), float(p3m_grid_density)))
+    out = dict(rc=d.rc, skin=d.skin, smoothing_width=d.smoothing_width, eta=d.eta,
+               kmode=d.kmode, kc=d.kc, order=d.order, nalias=d.nalias, ik=d.ik,
+               include_lj_correction=d.include_lj_correction, geometric_sigma=d.geometric_sigma)
+    if p3m:
+        out["mesh"] = tuple(d.mesh)
+    return out
+
+
+def derive_sites(natom, bonds_of):
+    """ElecLJ::init_sites for atom sites (mm_derive_sites).  bonds_of[a] is
+    the neighbour list of atom a in Atom::neighbors order.  Returns
+    (atom_of_site, groups [ngroup, 2], excluded pairs [nexcl, 2], ncross)."""
+    start = np.zeros(natom + 1, dtype=np.int32)
+    for a in range(natom):
+        start[a + 1] = start[a] + len(bonds_of[a])
+    nbr = np.array([b for a in range(natom) for b in bonds_of[a]] or [0], dtype=np.int32)
+    atom_of_site = np.zeros(natom, dtype=np.int32)
+    group = np.zeros(2 * natom, dtype=np.int32)
+    ngroup, nexcl, ncross = C.c_int(0), C.c_long(0), C.c_long(0)
+    args = lambda ex, cap: (natom, _ptr(start), _ptr(nbr), _ptr(atom_of_site), _ptr(group),
+                            C.byref(ngroup), _ptr(ex), cap, C.byref(nexcl), C.byref(ncross))
+    _check(lib().mm_derive_sites(*args(None, 0)))
+    excl = np.zeros(2 * max(nexcl.value, 1), dtype=np.int32)
+    _check(lib().mm_derive_sites(*args(excl, nexcl.value)))
+    return (atom_of_site, group[:2 * ngroup.value].reshape(-1, 2),
+            excl[:2 * nexcl.value].reshape(-1, 2), ncross.value)
 
 
 def _check(rc):

@@ -658,6 +658,125 @@ void orc_rspace_phi_at_gridpoints(int nsite, int ngrid, const double *rsite,
 }
 
 /* ------------------------------------------------------------------ */
+/* Non-periodic real space                                             */
+/* ------------------------------------------------------------------ */
+
+/* rspace.c:61-64 (screening inside rs), else 1/r and 1/r^3 */
+static void np_coulomb(double r2, double rs, double *b0, double *b1)
+{
+  if (r2 < rs * rs) {
+    *b1 = 1.0 / (rs * rs * rs);
+    *b0 = -0.5 * r2 * (*b1) + 1.5 / rs;
+  } else {
+    const double invr = 1.0 / sqrt(r2);
+    *b0 = invr;
+    *b1 = invr * invr * invr;
+  }
+}
+
+/* one site pair of RSpaceNonPeriodic: interact() (rspace.c:299-321) with
+ * coulomb_non_periodic (:66-102) and lj (:258-297), no smoothing/scaling */
+static void np_pair(int i, int j, const double *r, const int *site_mask, const double *q,
+                    const double *rscreen, const double *sig, const double *eps,
+                    int geometric, double *phi, double *evec, double *lj_ener, double *lj_force)
+{
+  const double x = r[3 * i] - r[3 * j], y = r[3 * i + 1] - r[3 * j + 1], z = r[3 * i + 2] - r[3 * j + 2];
+  const double r2 = x * x + y * y + z * z;
+  const int mask = site_mask[i] & site_mask[j];
+  if (mask & ORC_SITE_HAS_Q) {
+    double b0, b1;
+    np_coulomb(r2, rscreen[i] + rscreen[j], &b0, &b1);
+    const double qib1 = q[i] * b1, qjb1 = q[j] * b1;
+    phi[i] += q[j] * b0;
+    evec[3 * i] += qjb1 * x; evec[3 * i + 1] += qjb1 * y; evec[3 * i + 2] += qjb1 * z;
+    phi[j] += q[i] * b0;
+    evec[3 * j] -= qib1 * x; evec[3 * j + 1] -= qib1 * y; evec[3 * j + 2] -= qib1 * z;
+  }
+  if (mask & ORC_SITE_HAS_LJ) {
+    const double s2 = geometric ? sig[i] * sig[j] : sq(0.5 * (sig[i] + sig[j]));
+    const double s6 = s2 * s2 * s2;
+    const double b = 4 * sqrt(eps[i] * eps[j]) * s6, a = b * s6;
+    const double invr2 = 1.0 / r2, invr6 = invr2 * invr2 * invr2;
+    const double binvr6 = b * invr6, ainvr12 = a * invr6 * invr6;
+    const double du = 12 * ainvr12 * invr2 - 6 * binvr6 * invr2;
+    *lj_ener += ainvr12 - binvr6;
+    lj_force[3 * i] += du * x; lj_force[3 * j] -= du * x;
+    lj_force[3 * i + 1] += du * y; lj_force[3 * j + 1] -= du * y;
+    lj_force[3 * i + 2] += du * z; lj_force[3 * j + 2] -= du * z;
+  }
+}
+
+/* RSpaceNonPeriodic (rspace.c:560-640) without group exclusions or a 1-4
+ * list: every site pair of groups ig < jg, first sites first */
+void orc_rspace_nonperiodic(int nsite, int ngroup, const int *group, const double *r,
+                            const int *site_mask, const double *q, const double *rscreen,
+                            const double *sig, const double *eps, int geometric, double *phi,
+                            double *evec, double *lj_ener, double *lj_force)
+{
+  int ig, jg, i, j;
+  (void)nsite;
+  for (ig = 0; ig < ngroup; ig++)
+    for (jg = ig + 1; jg < ngroup; jg++) {
+      const int is = group[2 * ig], ie = group[2 * ig + 1], js = group[2 * jg], je = group[2 * jg + 1];
+      np_pair(is, js, r, site_mask, q, rscreen, sig, eps, geometric, phi, evec, lj_ener, lj_force);
+      for (i = is; i < ie; i++)
+        for (j = (i == is ? js + 1 : js); j < je; j++) {
+          if (!(site_mask[i] & site_mask[j])) continue;
+          np_pair(i, j, r, site_mask, q, rscreen, sig, eps, geometric, phi, evec, lj_ener, lj_force);
+        }
+    }
+}
+
+/* coulomb_non_periodic_phi (rspace.c:158-180) without smoothing/scaling */
+static void np_phi_pair(int i, int j, const double *r, const double *q, const double *rscreen,
+                        double *phi)
+{
+  const double x = r[3 * i] - r[3 * j], y = r[3 * i + 1] - r[3 * j + 1], z = r[3 * i + 2] - r[3 * j + 2];
+  double b0, b1;
+  np_coulomb(x * x + y * y + z * z, rscreen[i] + rscreen[j], &b0, &b1);
+  phi[i] += q[j] * b0;
+  phi[j] += q[i] * b0;
+}
+
+/* RSpaceNonPeriodicPhi (rspace.c:784-856): the first sites of two groups
+ * always interact; other pairs when their masks share any bit */
+void orc_rspace_nonperiodic_phi(int nsite, int ngroup, const int *group, const double *r,
+                                const int *site_mask, const double *q, const double *rscreen,
+                                double *phi)
+{
+  int ig, jg, i, j;
+  (void)nsite;
+  for (ig = 0; ig < ngroup; ig++)
+    for (jg = ig + 1; jg < ngroup; jg++) {
+      const int is = group[2 * ig], ie = group[2 * ig + 1], js = group[2 * jg], je = group[2 * jg + 1];
+      np_phi_pair(is, js, r, q, rscreen, phi);
+      for (i = is; i < ie; i++)
+        for (j = (i == is ? js + 1 : js); j < je; j++) {
+          if (!(site_mask[i] & site_mask[j])) continue;
+          np_phi_pair(i, j, r, q, rscreen, phi);
+        }
+    }
+}
+
+/* RSpaceNonPeriodicPhiAtGridpoints (rspace.c:858-890) */
+void orc_rspace_nonperiodic_phi_at_gridpoints(int nsite, int ngrid, const double *rsite,
+                                              const double *rgrid, const int *site_mask,
+                                              const double *q, const double *rscreen, double *phi)
+{
+  int i, j;
+  for (i = 0; i < nsite; i++) {
+    if (!site_mask[i]) continue;
+    for (j = 0; j < ngrid; j++) {
+      const double x = rsite[3 * i] - rgrid[3 * j], y = rsite[3 * i + 1] - rgrid[3 * j + 1],
+                   z = rsite[3 * i + 2] - rgrid[3 * j + 2];
+      double b0, b1;
+      np_coulomb(x * x + y * y + z * z, rscreen[i], &b0, &b1);
+      phi[j] += q[i] * b0;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* Plain Ewald k-space                                                 */
 /* ------------------------------------------------------------------ */
 

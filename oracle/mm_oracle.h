@@ -91,6 +91,20 @@ void orc_rspace_ewald_phi(int nsite, int ngroup, const int *group, const double 
 
 /* RSpaceEwaldPhiAtGridpoints (rspace.c:892-967): phi at ngrid points
  * (AoS rgrid) from all sites with any mask bit.  Accumulated (+=). */
+/* ---- Non-periodic real space (rspace.c:560-640, 784-890) ----
+ * no group exclusions, no 1-4 list; rscreen per site (zeros: no screening).
+ * Accumulated (+=). */
+void orc_rspace_nonperiodic(int nsite, int ngroup, const int *group, const double *r,
+                            const int *site_mask, const double *q, const double *rscreen,
+                            const double *sig, const double *eps, int geometric, double *phi,
+                            double *evec, double *lj_ener, double *lj_force);
+void orc_rspace_nonperiodic_phi(int nsite, int ngroup, const int *group, const double *r,
+                                const int *site_mask, const double *q, const double *rscreen,
+                                double *phi);
+void orc_rspace_nonperiodic_phi_at_gridpoints(int nsite, int ngrid, const double *rsite,
+                                              const double *rgrid, const int *site_mask,
+                                              const double *q, const double *rscreen, double *phi);
+
 void orc_rspace_phi_at_gridpoints(int nsite, int ngrid, const double *rsite,
                                   const double *rgrid, double boxl, double rlo2, double rhi2,
                                   const int *site_mask, const double *q, double eta,

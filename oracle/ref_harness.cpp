@@ -10,6 +10,8 @@
  *   PairBinary_* (pbin.c:15-137)
  *   RSpaceChebyshev_new (rspace.c:25-37), RSpaceEwald (rspace.c:382-558)
  *   kspace_new / kspace_field (kspace.c:35-174)
+ *   RSpaceNonPeriodic, RSpaceNonPeriodicPhi, RSpaceNonPeriodicPhiAtGridpoints
+ *   (rspace.c:560-640, 784-890)
  */
 #include "nbrlist.hpp"
 #include "bcond.hpp"
@@ -194,6 +196,31 @@ int ref_kspace_count(double kc)
   const int n = kspace_number_of_wavevectors(k);
   kspace_delete(k);
   return n;
+}
+
+/* RSpaceNonPeriodic (rspace.c:560-640), RSpaceNonPeriodicPhi (:784-856) and
+ * RSpaceNonPeriodicPhiAtGridpoints (:858-890) for a cluster: exclusions are
+ * the given site pairs (inside groups), no group exclusions, no 1-4 list;
+ * rscreen per site.  out: phi, evec (3), lj_force (3) per site; phi_only
+ * per site; grid_phi per point; lj_energy. */
+void ref_nonperiodic(int nsite, int ngroup, const int *group, const double *r, int nexcl,
+                     const int *excl, const int *site_mask, const double *q,
+                     const double *rscreen, const double *sig, const double *eps, int geometric,
+                     int ngrid, const double *grid, double *phi, double *evec,
+                     double *lj_force, double *lj_energy, double *phi_only, double *grid_phi)
+{
+  struct PairBinary *excluded = pb_from(nsite, nexcl, excl);
+  struct PairBinary *gexcl = PairBinary_new(ngroup);
+  PairBinary_init(gexcl);
+  RSpaceNonPeriodic(nsite, ngroup, (const int(*)[2])group, (const cart_t *)r, gexcl, excluded,
+                    site_mask, q, phi, (cart_t *)evec, rscreen, sig, eps, geometric, lj_energy,
+                    (cart_t *)lj_force, 0, 1, 1);
+  RSpaceNonPeriodicPhi(nsite, ngroup, (const int(*)[2])group, (const cart_t *)r, gexcl, excluded,
+                       site_mask, q, phi_only, rscreen, 0, 1);
+  RSpaceNonPeriodicPhiAtGridpoints(nsite, ngrid, (const cart_t *)r, (const cart_t *)grid,
+                                   site_mask, q, grid_phi, rscreen);
+  PairBinary_delete(excluded);
+  PairBinary_delete(gexcl);
 }
 
 /* PairBinary known-answer sequence of pbin.c:176-213: rows (rowptr/cols)

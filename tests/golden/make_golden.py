@@ -23,6 +23,11 @@ the reference's outputs):
                      RSpaceEwaldPhi phi and RSpaceEwald phi / evec there
   ref_esp_grid.npz   RSpaceEwaldPhiAtGridpoints on the 648-site box at 400
                      seeded points (inside and outside the central box)
+  ref_nonperiodic.npz  RSpaceNonPeriodic (phi, evec, lj_force, lj_energy),
+                     RSpaceNonPeriodicPhi and RSpaceNonPeriodicPhiAtGridpoints
+                     on a 40-water + 4-ion cluster with screening radii on the
+                     ions and one LJ-only charged ion (the phi variant's pair
+                     rule), at 120 seeded points
 """
 import hashlib
 import json
@@ -71,12 +76,24 @@ def write_esp_fixture():
     np.savez_compressed(os.path.join(HERE, "ref_esp_grid.npz"), grid=grid, phi=phi)
 
 
+def write_nonperiodic_fixture():
+    """RSpaceNonPeriodic, RSpaceNonPeriodicPhi and
+    RSpaceNonPeriodicPhiAtGridpoints on a cluster (ref_nonperiodic.npz)."""
+    s, rscreen = O.cluster_system()
+    grid = O.cluster_grid(s)
+    out = O.ref_nonperiodic_parts(s, rscreen, grid)
+    np.savez_compressed(os.path.join(HERE, "ref_nonperiodic.npz"), r=s.r, rscreen=rscreen,
+                        grid=grid, phi=out[0], evec=out[1], lj_force=out[2],
+                        lj_energy=np.array([out[3]]), phi_only=out[4], grid_phi=out[5])
+
+
 def main():
     lib = O.ref()
     if lib is None:
         raise SystemExit("the reference build (oracle/_ref/libref.so) is not available here")
     write_phi_fixture()
     write_esp_fixture()
+    write_nonperiodic_fixture()
     s, p = W.config("water648_ewald")
     gr = O.first_sites(s)
     nl = O.ref_nl_pairs(gr, s.boxl, p["rc"], p["skin"])

@@ -169,3 +169,160 @@ def test_p3m_dropin(gpu_lib):
     ophi, oev, _, _ = O.p3m_parts(s, p)
     print("dropin p3m", rel(phi, ophi), rel(ev, oev))
     assert rel(phi, ophi) < 1e-5 and rel(ev, oev) < 1e-5
+
+
+def np_lib():
+    L = lib()
+    L.RSpaceNonPeriodic.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P, P, I, P, P, P, D, D]
+    L.RSpaceNonPeriodicPhi.argtypes = [I, I, P, P, P, P, P, P, P, P, P, D]
+    L.RSpaceNonPeriodicPhiAtGridpoints.argtypes = [I, I, P, P, P, P, P, P]
+    return L
+
+
+def test_nonperiodic_dropins(gpu_lib):
+    """RSpaceNonPeriodic, RSpaceNonPeriodicPhi and
+    RSpaceNonPeriodicPhiAtGridpoints (FP64 on the device) against the oracle,
+    which equals the compiled reference bit for bit on this cluster: 1e-12
+    relative (summation order differs); outputs accumulate (+=)."""
+    L = np_lib()
+    s, rs = O.cluster_system()
+    grid = O.cluster_grid(s)
+    n, ng = s.nsite, len(grid)
+    r = np.ascontiguousarray(s.r)
+    group = np.ascontiguousarray(s.group)
+    phi = np.full(n, 0.5); ev = np.zeros((n, 3)); ljf = np.zeros((n, 3)); lje = np.full(1, 2.0)
+    phi_only = np.zeros(n); gphi = np.full(ng, -1.0)
+    print("[refapi] non-periodic drop-ins on a 124-site cluster", flush=True)
+    L.RSpaceNonPeriodic(n, s.ngroup, group.ctypes.data_as(P), r.ctypes.data_as(P), None, None,
+                        s.site_mask.ctypes.data_as(P), s.q.ctypes.data_as(P),
+                        phi.ctypes.data_as(P), ev.ctypes.data_as(P), rs.ctypes.data_as(P),
+                        s.sig.ctypes.data_as(P), s.eps.ctypes.data_as(P), 0,
+                        lje.ctypes.data_as(P), ljf.ctypes.data_as(P), None, 1.0, 1.0)
+    L.RSpaceNonPeriodicPhi(n, s.ngroup, group.ctypes.data_as(P), r.ctypes.data_as(P), None,
+                           None, s.site_mask.ctypes.data_as(P), s.q.ctypes.data_as(P),
+                           phi_only.ctypes.data_as(P), rs.ctypes.data_as(P), None, 1.0)
+    g = np.ascontiguousarray(grid)
+    L.RSpaceNonPeriodicPhiAtGridpoints(n, ng, r.ctypes.data_as(P), g.ctypes.data_as(P),
+                                       s.site_mask.ctypes.data_as(P), s.q.ctypes.data_as(P),
+                                       gphi.ctypes.data_as(P), rs.ctypes.data_as(P))
+    o = O.nonperiodic_parts(s, rs, grid)
+    errs = [rel(phi - 0.5, o[0]), rel(ev, o[1]), rel(ljf, o[2]), abs(lje[0] - 2.0 - o[3]) / abs(o[3]),
+            rel(phi_only, o[4]), rel(gphi + 1.0, o[5])]
+    print("[refapi] non-periodic rel errors", errs)
+    assert max(errs) <= 1e-12
+
+
+def test_kspace_phi_dropin(gpu_lib):
+    """kspace_phi (kspace.c:176-251): the potential of kspace_field alone."""
+    L = lib()
+    L.kspace_phi.argtypes = [P, P, P, P, D, D, P]
+    s, p = W.config("water648_ewald")
+    k = L.kspace_new(p["kc"], s.nsite)
+    rl = O.recip_cubic(s.boxl)
+    phi = np.zeros(s.nsite)
+    L.kspace_phi(k, s.r.ctypes.data_as(P), s.q.ctypes.data_as(P), rl.ctypes.data_as(P),
+                 s.boxl ** 3, p["eta"], phi.ctypes.data_as(P))
+    L.kspace_delete(k)
+    ophi, _, _, _ = O.kspace_parts(s, p["kc"], p["eta"])
+    assert rel(phi, ophi) < 1e-11
+
+
+def p3m_handle(L, s, mesh, eta, order=5):
+    latv = np.array([s.boxl, 0, 0, 0, s.boxl, 0, 0, 0, s.boxl])
+    ng = np.zeros(3, dtype=np.int32)
+    pp = L.p3m_new(s.nsite, latv.ctypes.data_as(P), s.boxl / mesh, 2, 0, order,
+                   ng.ctypes.data_as(P))
+    assert list(ng) == [mesh] * 3
+    chi = np.zeros(1)
+    L.p3m_init(pp, eta, chi.ctypes.data_as(P))
+    return pp, chi[0]
+
+
+def p3m_run(L, pp, s, r=None, evec=True):
+    r = np.ascontiguousarray(s.r if r is None else r)
+    phi = np.zeros(s.nsite)
+    ev = np.zeros((s.nsite, 3)) if evec else None
+    L.p3m_field(pp, r.ctypes.data_as(P), s.q.ctypes.data_as(P), phi.ctypes.data_as(P),
+                ev.ctypes.data_as(P) if evec else None)
+    return phi, ev
+
+
+def test_p3m_field_phi_only(gpu_lib):
+    """p3m_field(..., evec = NULL) gives the potential alone (pppm.h:48-51)."""
+    L = lib()
+    s, p = W.config("water648_ewald")
+    pp, _ = p3m_handle(L, s, 24, 0.35)
+    phi, _ = p3m_run(L, pp, s, evec=False)
+    phi2, ev2 = p3m_run(L, pp, s)
+    L.p3m_delete(pp)
+    assert np.array_equal(phi, phi2)
+    ophi, oev, _, _ = O.p3m_parts(s, dict(mesh=(24, 24, 24), eta=0.35, order=5))
+    assert rel(phi, ophi) < 1e-5 and rel(ev2, oev) < 1e-5
+
+
+def test_p3m_scale_and_copy(gpu_lib):
+    """p3m_scale (pppm.c:643-664): the box times f and G / f is the P3M of
+    the scaled box with eta / f (G is homogeneous in k and eta), so the field
+    at scaled positions equals the oracle's for that box and eta.
+    p3m_copy (pppm.c:294-311) to fewer sites keeps the influence function."""
+    L = lib()
+    L.p3m_scale.argtypes = [P, D]
+    L.p3m_copy.restype = P
+    L.p3m_copy.argtypes = [P, I]
+    s, p = W.config("water648_ewald")
+    eta, f = 0.35, 1.03
+    pp, _ = p3m_handle(L, s, 24, eta)
+    L.p3m_scale(pp, f)
+    t = W.System(**{k: getattr(s, k) for k in s.__dataclass_fields__})
+    t.boxl, t.r = s.boxl * f, s.r * f
+    phi, ev = p3m_run(L, pp, t)
+    ophi, oev, _, _ = O.p3m_parts(t, dict(mesh=(24, 24, 24), eta=eta / f, order=5))
+    print("[refapi] p3m_scale rel", rel(phi, ophi), rel(ev, oev))
+    assert rel(phi, ophi) < 1e-5 and rel(ev, oev) < 1e-5
+    # copy to the first 27 waters, at the unscaled box again
+    small = W.subset(s, np.arange(27))
+    L.p3m_scale(pp, 1 / f)
+    cp = L.p3m_copy(pp, small.nsite)
+    L.p3m_delete(pp)
+    phi_c, ev_c = p3m_run(L, cp, small)
+    L.p3m_delete(cp)
+    sphi, sev, _, _ = O.p3m_parts(small, dict(mesh=(24, 24, 24), eta=eta, order=5))
+    print("[refapi] p3m_copy rel", rel(phi_c, sphi), rel(ev_c, sev))
+    assert rel(phi_c, sphi) < 1e-5 and rel(ev_c, sev) < 1e-5
+
+
+def test_p3m_optimize_finds_the_minimum(gpu_lib):
+    """p3m_optimize (pppm.c:667-716) minimises chi_r^2 + chi_p3m^2 over eta
+    (the reference uses GSL's simplex, absent here): the returned eta is
+    within 1e-3 relative of the minimum of a fine scan of the same function
+    computed with the oracle's influence function, its value is the scan's
+    minimum to 1e-4 (the simplex stops at an eta step of 1e-4, where the
+    function is flat to second order), and the returned estimates are those
+    of the returned eta."""
+    L = lib()
+    L.p3m_optimize.argtypes = [P, D, P, P, P]
+    s, p = W.config("water648_ewald")
+    rc, mesh = 9.0, 20
+    pp, _ = p3m_handle(L, s, mesh, 0.3)
+    eta, cr, ck = np.zeros(1), np.zeros(1), np.zeros(1)
+    L.p3m_optimize(pp, rc, eta.ctypes.data_as(P), cr.ctypes.data_as(P), ck.ctypes.data_as(P))
+    L.p3m_delete(pp)
+    V = s.boxl ** 3
+
+    def f(x):
+        chi_r = 2 / np.sqrt(rc / V ** (1 / 3)) * np.exp(-(rc * x) ** 2)
+        _, chi_k = O.green((mesh,) * 3, s.boxl, x, 5)
+        return chi_r ** 2 + chi_k ** 2, chi_r, chi_k
+
+    xs = np.linspace(0.8 * eta[0], 1.2 * eta[0], 81)
+    vals = [f(x)[0] for x in xs]
+    k = int(np.argmin(vals))
+    assert 0 < k < len(xs) - 1, "the scan brackets the minimum"
+    fine = np.linspace(xs[k - 1], xs[k + 1], 41)
+    fv = [f(x)[0] for x in fine]
+    x_min = fine[int(np.argmin(fv))]
+    fo, cro, cko = f(eta[0])
+    print(f"[refapi] p3m_optimize eta {eta[0]:.6f}, scan minimum {x_min:.6f}")
+    assert abs(eta[0] - x_min) <= 1e-3 * x_min
+    assert fo <= min(fv) * (1 + 1e-4)
+    assert abs(cr[0] - cro) <= 1e-12 * cro and abs(ck[0] - cko) <= 1e-9 * cko
