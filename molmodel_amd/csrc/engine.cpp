@@ -765,6 +765,9 @@ int engine_set_params(Engine &e, int s, const double *q, const double *sig, cons
 void engine_drop_graph(Engine &e) {
   if (e.gexec) { hipGraphExecDestroy(e.gexec); e.gexec = nullptr; }
   if (e.graph) { hipGraphDestroy(e.graph); e.graph = nullptr; }
+  if (e.gexecK) { hipGraphExecDestroy(e.gexecK); e.gexecK = nullptr; }
+  if (e.graphK) { hipGraphDestroy(e.graphK); e.graphK = nullptr; }
+  e.graphK_steps = 0;
 }
 
 int engine_grow_bins(Engine &e) {
@@ -794,8 +797,7 @@ int engine_grow_bins(Engine &e) {
 }
 
 void engine_destroy(Engine &e) {
-  if (e.gexec) hipGraphExecDestroy(e.gexec);
-  if (e.graph) hipGraphDestroy(e.graph);
+  engine_drop_graph(e);
   void *ptrs[] = {e.d_sp, e.d_gfirst, e.d_gns, e.d_gsys, e.d_gexcl, e.d_ginfo, e.d_q, e.d_mass,
                   e.d_r, e.d_v, e.d_f, e.d_r0, e.d_gcell, e.d_cell_count, e.d_cell_start,
                   e.d_cell_fill, e.d_order, e.d_gslot, e.d_last, e.d_sten, e.d_nbr, e.d_nnb, e.d_rebuild,

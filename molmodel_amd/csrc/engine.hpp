@@ -167,6 +167,10 @@ struct Engine {
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // several consecutive steps captured as one graph (fewer graph launches)
+  hipGraph_t graphK = nullptr;
+  hipGraphExec_t gexecK = nullptr;
+  int graphK_steps = 0;
   int graph_mode = -1;
 };
 
