@@ -43,6 +43,54 @@ int nl_nmax(int ncell, double boxl, double rc, double skin) {
 // pair rows reach, plus the zero offset for same-cell pairs; offsets whose
 // cells cannot hold a pair closer than rc+skin are dropped (they add no
 // pairs), which leaves the pair set unchanged.
+// Index of (x, y, z) along the Hilbert curve through a 2^bits cube, by
+// J. Skilling's transpose method ("Programming the Hilbert curve", AIP
+// Conf. Proc. 707, 381 (2004)): the axes are transformed to the curve's
+// transposed index, whose bits are then interleaved.
+static unsigned long long hilbert_index(unsigned x, unsigned y, unsigned z, int bits) {
+  unsigned X[3] = {x, y, z};
+  const unsigned M = 1u << (bits - 1);
+  for (unsigned Q = M; Q > 1; Q >>= 1) {
+    const unsigned P = Q - 1;
+    for (int i = 0; i < 3; i++) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const unsigned t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 3; i++) X[i] ^= X[i - 1];
+  unsigned t = 0;
+  for (unsigned Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; i++) X[i] ^= t;
+  unsigned long long key = 0;
+  for (int b = bits - 1; b >= 0; b--)
+    for (int i = 0; i < 3; i++) key = (key << 1) | ((X[i] >> b) & 1u);
+  return key;
+}
+
+// storage position of every cell index of an n^3 grid along the Hilbert
+// curve (rank), and the cell index at every position (unrank)
+void hilbert_cell_order(int n, std::vector<int> &rank, std::vector<int> &unrank) {
+  int bits = 1;
+  while ((1 << bits) < n) bits++;
+  const int nc = n * n * n;
+  std::vector<std::pair<unsigned long long, int>> key(nc);
+  for (int c = 0; c < nc; c++)
+    key[c] = {hilbert_index((unsigned)(c / (n * n)), (unsigned)((c / n) % n), (unsigned)(c % n), bits), c};
+  std::sort(key.begin(), key.end());
+  rank.assign(nc, 0);
+  unrank.assign(nc, 0);
+  for (int k = 0; k < nc; k++) {
+    rank[key[k].second] = k;
+    unrank[k] = key[k].second;
+  }
+}
+
 static std::vector<int4> nl_stencil(int ncell, double boxl, double rc, double skin) {
   const double lc = boxl * (1.0 / ncell);
   const double inv = 1 / lc;
@@ -561,6 +609,16 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   MM_TRY(dalloc(&e.d_cell_count, e.ncell_total + 1));
   MM_TRY(dalloc(&e.d_cell_start, e.ncell_total + 1));
   MM_TRY(dalloc(&e.d_cell_fill, e.ncell_total + 1));
+  {
+    std::vector<int> rank_all, unrank_all, rk, ur;
+    for (auto &P : e.hsp) {
+      hilbert_cell_order(P.ncell, rk, ur);
+      rank_all.insert(rank_all.end(), rk.begin(), rk.end());
+      unrank_all.insert(unrank_all.end(), ur.begin(), ur.end());
+    }
+    MM_TRY(dalloc(&e.d_cell_rank, rank_all.size())); MM_TRY(dput(e.d_cell_rank, rank_all));
+    MM_TRY(dalloc(&e.d_cell_unrank, unrank_all.size())); MM_TRY(dput(e.d_cell_unrank, unrank_all));
+  }
   MM_TRY(dalloc(&e.d_order, G));
   {
     std::vector<int4> sl(std::max(G, 1), make_int4(0, 0, 0, 0));
@@ -800,6 +858,7 @@ void engine_destroy(Engine &e) {
   engine_drop_graph(e);
   void *ptrs[] = {e.d_sp, e.d_gfirst, e.d_gns, e.d_gsys, e.d_gexcl, e.d_ginfo, e.d_q, e.d_mass,
                   e.d_r, e.d_v, e.d_f, e.d_r0, e.d_gcell, e.d_cell_count, e.d_cell_start,
+                  e.d_cell_rank, e.d_cell_unrank,
                   e.d_cell_fill, e.d_order, e.d_gslot, e.d_last, e.d_sten, e.d_nbr, e.d_nnb, e.d_rebuild,
                   e.d_err, e.d_gpos, e.d_spk, e.d_gmeta, e.d_pf, e.d_plj, e.d_gacc, e.d_tab,
                   e.d_ljt, e.d_kres, e.d_kvec, e.d_ksc, e.d_mesh, e.d_cmesh, e.d_green, e.d_twid,

@@ -73,6 +73,11 @@ struct Engine {
   // neighbor list
   int ncell_total = 0;
   int *d_gcell = nullptr, *d_cell_count = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr;
+  // cells are stored in Hilbert-curve order: d_cell_rank maps a cell index
+  // (ix*n + iy)*n + iz to its storage position, d_cell_unrank back (both per
+  // system, relative to SysParams::cell0); groups sorted by stored cell are
+  // then spatially compact in every contiguous range
+  int *d_cell_rank = nullptr, *d_cell_unrank = nullptr;
   int *d_order = nullptr;     // sorted gs -> original group
   int4 *d_gslot = nullptr;    // per sorted slot: group, first site, site count, exclusion
                               // mask (written with each neighbour-list build)

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) k_nl_assign(
     const double *__restrict__ r, int nsite_total, int *__restrict__ rebuild,
     int *__restrict__ gcell, double *__restrict__ last, int *__restrict__ cell_count,
     unsigned long long *__restrict__ builds, int *__restrict__ err,
-    const double *__restrict__ part, int maxblk, int from_part) {
+    const double *__restrict__ part, int maxblk, int from_part, const int *__restrict__ cell_rank) {
 #pragma clang fp contract(off)
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
@@ -86,7 +86,8 @@ __global__ void __launch_bounds__(256) k_nl_assign(
     const int ix = (int)floor(x / P.lc + 0.5);
     const int iy = (int)floor(y / P.lc + 0.5);
     const int iz = (int)floor(z / P.lc + 0.5);
-    c += (mod_nonneg(ix, n) * n + mod_nonneg(iy, n)) * n + mod_nonneg(iz, n);
+    // stored at the cell's position along the Hilbert curve
+    c += cell_rank[P.cell0 + (mod_nonneg(ix, n) * n + mod_nonneg(iy, n)) * n + mod_nonneg(iz, n)];
   } else {
     atomicOr(err, ERR_NAN);   // reported; the group is parked in cell 0
   }
@@ -174,7 +175,8 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
     const double *__restrict__ r, int N, const double *__restrict__ q,
     const int *__restrict__ gfirst, const int *__restrict__ gns, const int *__restrict__ ginfo,
     const double4 *__restrict__ gpos, float4 *__restrict__ spk, int2 *__restrict__ gmeta,
-    const int *__restrict__ gexcl, int4 *__restrict__ gslot) {
+    const int *__restrict__ gexcl, int4 *__restrict__ gslot, const int *__restrict__ cell_rank,
+    const int *__restrict__ cell_unrank) {
 #pragma clang fp contract(off)
   __shared__ int c_start[4][NL_SEG], c_off[4][NL_SEG + 1], c_cell[4][NL_SEG];
   const int s = blockIdx.y;
@@ -203,7 +205,9 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
       continue;
     }
     const int n = P.ncell;
-    const int A = gcell[g] - P.cell0;
+    // A, B: cell indices (ix*n + iy)*n + iz, as the reference numbers them;
+    // cells are stored at their Hilbert positions (cell_rank)
+    const int A = cell_unrank[gcell[g]];
     const int ax = A / (n * n), ay = (A / n) % n, az = A % n;
     const double4 pg = gpos[gs];
     const double L = P.L, invL = 1 / P.L;
@@ -259,8 +263,9 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
         const int4 o = sten[P.sten0 + t];
         const int bxx = (ax + o.x) % n, byy = (ay + o.y) % n, bzz = (az + o.z) % n;
         B = (bxx * n + byy) * n + bzz;
-        start = cell_start[P.cell0 + B];
-        nb = cell_start[P.cell0 + B + 1] - start;
+        const int Bs = P.cell0 + cell_rank[P.cell0 + B];
+        start = cell_start[Bs];
+        nb = cell_start[Bs + 1] - start;
         // a cell whose box (cell b holds x in [(b - 1/2) lc, (b + 1/2) lc)
         // mod L) lies farther than rc + skin from this group holds no pair;
         // the margin covers the rounding of the cell assignment
@@ -380,7 +385,7 @@ int launch_nl_build(Engine &e, int force, int from_partials) {
   const double *r = e.d_r;
   hipLaunchKernelGGL(k_nl_assign, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_gfirst, r, e.nsite,
                      e.d_rebuild, e.d_gcell, e.d_last, e.d_cell_count, e.d_nlb, e.d_err,
-                     e.d_fpart, e.nblk_part, from_partials);
+                     e.d_fpart, e.nblk_part, from_partials, e.d_cell_rank);
   hipLaunchKernelGGL(k_nl_scan, dim3(1, e.nsys), dim3(1024), 0, e.stream, e.d_sp, e.d_rebuild,
                      e.d_cell_count, e.d_cell_start, e.d_cell_fill);
   hipLaunchKernelGGL(k_nl_scatter, gg, dim3(tb), 0, e.stream, e.d_sp, e.d_rebuild, e.d_gcell,
@@ -394,7 +399,7 @@ int launch_nl_build(Engine &e, int force, int from_partials) {
                      e.stream, e.d_sp, e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start,
                      e.d_sten, e.nbcap, e.d_nbr, e.d_nnb, e.d_err, e.d_r, e.nsite, e.d_q,
                      e.d_gfirst, e.d_gns, e.d_ginfo, e.d_gpos, e.d_spk, e.d_gmeta, e.d_gexcl,
-                     e.d_gslot);
+                     e.d_gslot, e.d_cell_rank, e.d_cell_unrank);
   MM_HIP(hipGetLastError());
   return 0;
 }
