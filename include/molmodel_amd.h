@@ -117,6 +117,14 @@ int mm_derive_sites(int natom, const int *nbr_start, const int *nbr, int *atom_o
                     int *group, int *ngroup, int *excl, long maxexcl, long *nexcl,
                     long *ncross);
 
+/* The buffer layout mm_engine_create would give this batch, checked on the
+ * host (no device needed): every system's slice of each concatenated device
+ * buffer (sites, groups, cells, stencils, erfc and LJ tables, wave vectors,
+ * meshes, influence functions, twiddles, spreading bins) lies inside the
+ * buffer and the slices are disjoint.  0, or MM_ERR_ARG with the first
+ * violation (or the input error engine creation would report). */
+int mm_check_layout(int nsys, const mm_system_desc *sys);
+
 typedef struct mm_engine mm_engine;
 
 /* energy slots returned per system by mm_get_energies */

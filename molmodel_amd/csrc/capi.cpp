@@ -33,17 +33,9 @@ int mm_device_count(void) {
   return n;
 }
 
-int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out) {
-  if (!out || nsys <= 0 || !sys) {
-    set_last_error("mm_engine_create: bad arguments");
-    return MM_ERR_ARG;
-  }
-  *out = nullptr;
-  if (mm_device_count() <= 0) {
-    set_last_error("mm_engine_create: no HIP device");
-    return MM_ERR_HIP;
-  }
-  std::vector<SystemInput> in(nsys);
+// the engine's inputs from the C descriptors
+static int desc_inputs(int nsys, const mm_system_desc *sys, std::vector<SystemInput> &in) {
+  in.assign(nsys, SystemInput());
   for (int s = 0; s < nsys; s++) {
     const mm_system_desc &d = sys[s];
     SystemInput &S = in[s];
@@ -82,6 +74,21 @@ int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out) {
     S.include_lj_correction = d.include_lj_correction;
     S.geometric_sigma = d.geometric_sigma;
   }
+  return MM_OK;
+}
+
+int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out) {
+  if (!out || nsys <= 0 || !sys) {
+    set_last_error("mm_engine_create: bad arguments");
+    return MM_ERR_ARG;
+  }
+  *out = nullptr;
+  if (mm_device_count() <= 0) {
+    set_last_error("mm_engine_create: no HIP device");
+    return MM_ERR_HIP;
+  }
+  std::vector<SystemInput> in;
+  TRY(desc_inputs(nsys, sys, in));
   mm_engine *m = new mm_engine();
   int rc = engine_create(m->e, in);
   if (rc) {
@@ -90,6 +97,20 @@ int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out) {
     return rc;
   }
   *out = m;
+  return MM_OK;
+}
+
+int mm_check_layout(int nsys, const mm_system_desc *sys) {
+  if (nsys <= 0 || !sys) {
+    set_last_error("mm_check_layout: bad arguments");
+    return MM_ERR_ARG;
+  }
+  std::vector<SystemInput> in;
+  TRY(desc_inputs(nsys, sys, in));
+  Engine e;
+  HostLayout h;
+  if (engine_layout(e, in, h)) return MM_ERR_ARG;
+  if (engine_check_layout(e, h)) return MM_ERR_ARG;
   return MM_OK;
 }
 

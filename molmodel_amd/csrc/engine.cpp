@@ -371,18 +371,23 @@ static double lj_tail(const SystemInput &s) {
                        224 * rhi2 * rlo5 + 192 * rhi * rlo6 + 32 * rlo7)));
 }
 
-int engine_create(Engine &e, const std::vector<SystemInput> &in) {
+// Host half of engine creation: every system's parameters and its slices of
+// the concatenated device buffers (sites, groups, cells, stencils, tables,
+// wave vectors, meshes, twiddles, spreading bins), and the host copies of
+// the buffers' contents.  Nothing here touches a device.
+int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h) {
   e.in = in;
   e.nsys = (int)in.size();
   MM_REQ(e.nsys > 0, "no systems");
-  std::vector<int> gfirst, gns, gsys, ginfo, gexcl;
-  std::vector<double> q, mass;
-  std::vector<float4> tab;
-  std::vector<float2> ljt;
-  std::vector<int4> sten;
-  std::vector<double> kvec;
-  std::vector<float> green;
-  std::vector<float2> twid;
+  std::vector<int> &gfirst = h.gfirst, &gns = h.gns, &gsys = h.gsys, &ginfo = h.ginfo,
+                   &gexcl = h.gexcl;
+  std::vector<double> &q = h.q, &mass = h.mass;
+  std::vector<float4> &tab = h.tab;
+  std::vector<float2> &ljt = h.ljt;
+  std::vector<int4> &sten = h.sten;
+  std::vector<double> &kvec = h.kvec;
+  std::vector<float> &green = h.green;
+  std::vector<float2> &twid = h.twid;
   int site0 = 0, group0 = 0, cell0 = 0, k0 = 0, bin0 = 0;
   long mesh0 = 0, cmesh0 = 0, green0 = 0;
   int maxnb = 64;
@@ -570,13 +575,94 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   e.green_total = green0;
   e.nbin_total = bin0;
   e.nbcap = (maxnb + 15) / 16 * 16;
-  const int N = e.nsite, G = e.ngroup;
   // partial-sum slots: finish blocks and x-pass blocks
   int nb = (e.maxg + 63) / 64;
   for (auto &P : e.hsp)
     if (P.kmode == 1)
       nb = std::max(nb, (P.ny * (P.nz / 2 + 1) + P3M_FFT_LINES - 1) / P3M_FFT_LINES);
   e.nblk_part = nb;
+  for (auto &P : e.hsp) {
+    P.water = P.ngroup > 0;
+    for (int g = P.group0; g < P.group0 + P.ngroup; g++)
+      if (gns[g] != 3 || (ginfo[g] & 0x030303) != WATER_MASKS) P.water = 0;
+    e.any_water |= P.water;
+  }
+  return 0;
+}
+
+// Every system's slice of each concatenated buffer lies inside the buffer,
+// and the slices are disjoint.
+int engine_check_layout(const Engine &e, const HostLayout &h) {
+  struct Slice { long at, n; int sys; };
+  auto check = [](const char *what, std::vector<Slice> v, long total) -> int {
+    std::sort(v.begin(), v.end(), [](const Slice &a, const Slice &b) { return a.at < b.at; });
+    long end = 0;
+    for (auto &x : v) {
+      if (x.n == 0) continue;
+      if (x.at < end || x.at + x.n > total || x.n < 0) {
+        set_last_error(std::string("layout: ") + what + " of system " + std::to_string(x.sys) +
+                       " [" + std::to_string(x.at) + ", " + std::to_string(x.at + x.n) +
+                       ") overlaps another or leaves the buffer of " + std::to_string(total));
+        return -1;
+      }
+      end = x.at + x.n;
+    }
+    return 0;
+  };
+  std::vector<Slice> site, group, cell, st, tb, lj, kv, mesh, cmesh, gr, tw, bin;
+  for (int s = 0; s < e.nsys; s++) {
+    const SysParams &P = e.hsp[s];
+    site.push_back({P.site0, P.nsite, s});
+    group.push_back({P.group0, P.ngroup, s});
+    cell.push_back({P.cell0, (long)P.ncell * P.ncell * P.ncell, s});
+    st.push_back({P.sten0, P.nsten, s});
+    tb.push_back({P.tab0, P.ntabA + P.ntabB, s});
+    lj.push_back({P.lj0, (long)P.ntype * P.ntype, s});
+    if (P.kmode == 0) kv.push_back({4L * P.k0, 4L * P.nk, s});
+    if (P.kmode == 1) {
+      const long nzh = P.nz / 2 + 1;
+      mesh.push_back({P.mesh0, (long)P.nx * P.ny * P.nz, s});
+      cmesh.push_back({P.cmesh0, (long)P.nx * P.ny * nzh, s});
+      gr.push_back({P.green0, (long)P.nx * P.ny * nzh, s});
+      tw.push_back({P.twid0, 3L * 128, s});
+      bin.push_back({P.bin0, (long)P.nbx * P.nby, s});
+      if (P.bx * P.nbx != P.nx || P.by * P.nby != P.ny || P.bx * P.by * P.nz > 4096) {
+        set_last_error("layout: spreading blocks of system " + std::to_string(s) +
+                       " do not tile its mesh");
+        return -1;
+      }
+    }
+  }
+  MM_TRY(check("sites", site, e.nsite));
+  MM_TRY(check("groups", group, e.ngroup));
+  MM_TRY(check("cells", cell, e.ncell_total));
+  MM_TRY(check("stencil", st, (long)h.sten.size()));
+  MM_TRY(check("erfc table", tb, (long)h.tab.size()));
+  MM_TRY(check("LJ table", lj, (long)h.ljt.size()));
+  MM_TRY(check("wave vectors", kv, (long)h.kvec.size()));
+  MM_TRY(check("mesh", mesh, e.mesh_total));
+  MM_TRY(check("half spectrum", cmesh, e.cmesh_total));
+  MM_TRY(check("influence function", gr, (long)h.green.size()));
+  MM_TRY(check("twiddles", tw, (long)h.twid.size()));
+  MM_TRY(check("spreading bins", bin, e.nbin_total));
+  MM_REQ((long)h.kvec.size() == 4L * e.nk_total, "layout: wave vector count");
+  MM_REQ((long)h.gfirst.size() == e.ngroup && (long)h.q.size() == e.nsite, "layout: site tables");
+  return 0;
+}
+
+int engine_create(Engine &e, const std::vector<SystemInput> &in) {
+  HostLayout h;
+  MM_TRY(engine_layout(e, in, h));
+  const std::vector<int> &gfirst = h.gfirst, &gns = h.gns, &gsys = h.gsys, &ginfo = h.ginfo,
+                         &gexcl = h.gexcl;
+  const std::vector<double> &q = h.q, &mass = h.mass;
+  const std::vector<float4> &tab = h.tab;
+  const std::vector<float2> &ljt = h.ljt;
+  const std::vector<int4> &sten = h.sten;
+  const std::vector<double> &kvec = h.kvec;
+  const std::vector<float> &green = h.green;
+  const std::vector<float2> &twid = h.twid;
+  const int N = e.nsite, G = e.ngroup;
 
   MM_HIP(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
   // MOLMODEL_AMD_SERIAL_KSPACE=1 keeps the reciprocal-space branch on the
@@ -586,12 +672,6 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
     MM_HIP(hipStreamCreateWithFlags(&e.stream2, hipStreamNonBlocking));
   MM_HIP(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
   MM_HIP(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
-  for (auto &P : e.hsp) {
-    P.water = P.ngroup > 0;
-    for (int g = P.group0; g < P.group0 + P.ngroup; g++)
-      if (gns[g] != 3 || (ginfo[g] & 0x030303) != WATER_MASKS) P.water = 0;
-    e.any_water |= P.water;
-  }
   MM_TRY(dalloc(&e.d_sp, e.nsys));
   MM_TRY(dput(e.d_sp, e.hsp));
   MM_TRY(dalloc(&e.d_gfirst, G)); MM_TRY(dput(e.d_gfirst, gfirst));

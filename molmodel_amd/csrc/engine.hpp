@@ -179,6 +179,19 @@ struct Engine {
   int graph_mode = -1;
 };
 
+// host copies of the concatenated per-system buffers engine_create uploads
+struct HostLayout {
+  std::vector<int> gfirst, gns, gsys, ginfo, gexcl;
+  std::vector<double> q, mass;
+  std::vector<float4> tab;
+  std::vector<float2> ljt;
+  std::vector<int4> sten;
+  std::vector<double> kvec;
+  std::vector<float> green;
+  std::vector<float2> twid;
+};
+int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h);
+int engine_check_layout(const Engine &e, const HostLayout &h);
 int engine_create(Engine &e, const std::vector<SystemInput> &in);
 // host helpers shared with the reference-named entry points
 void tensor_inverse(double *a, const double *b);
