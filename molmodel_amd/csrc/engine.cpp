@@ -705,6 +705,10 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
     for (int g = 0; g < G; g++) sl[g] = make_int4(g, gfirst[g], gns[g], gexcl[g]);
     MM_TRY(dalloc(&e.d_gslot, sl.size()));
     MM_TRY(dput(e.d_gslot, sl));
+    std::vector<int> inv(std::max(G, 1));
+    for (int g = 0; g < G; g++) inv[g] = g;
+    MM_TRY(dalloc(&e.d_gslot_inv, inv.size()));
+    MM_TRY(dput(e.d_gslot_inv, inv));
   }
   MM_TRY(dalloc(&e.d_last, 3 * (size_t)G));
   MM_TRY(dalloc(&e.d_sten, sten.size())); MM_TRY(dput(e.d_sten, sten));
@@ -939,7 +943,8 @@ void engine_destroy(Engine &e) {
   void *ptrs[] = {e.d_sp, e.d_gfirst, e.d_gns, e.d_gsys, e.d_gexcl, e.d_ginfo, e.d_q, e.d_mass,
                   e.d_r, e.d_v, e.d_f, e.d_r0, e.d_gcell, e.d_cell_count, e.d_cell_start,
                   e.d_cell_rank, e.d_cell_unrank,
-                  e.d_cell_fill, e.d_order, e.d_gslot, e.d_last, e.d_sten, e.d_nbr, e.d_nnb, e.d_rebuild,
+                  e.d_cell_fill, e.d_order, e.d_gslot, e.d_gslot_inv, e.d_last, e.d_sten, e.d_nbr,
+                  e.d_nnb, e.d_rebuild,
                   e.d_err, e.d_gpos, e.d_spk, e.d_gmeta, e.d_pf, e.d_plj, e.d_gacc, e.d_tab,
                   e.d_ljt, e.d_kres, e.d_kvec, e.d_ksc, e.d_mesh, e.d_cmesh, e.d_green, e.d_twid,
                   e.d_sgrid, e.d_sfrac, e.d_bin_fill,

@@ -79,6 +79,7 @@ struct Engine {
   // then spatially compact in every contiguous range
   int *d_cell_rank = nullptr, *d_cell_unrank = nullptr;
   int *d_order = nullptr;     // sorted gs -> original group
+  int *d_gslot_inv = nullptr; // per group: its sorted slot (inverse of d_order)
   int4 *d_gslot = nullptr;    // per sorted slot: group, first site, site count, exclusion
                               // mask (written with each neighbour-list build)
   double *d_last = nullptr;   // 3*ngroup first-site positions at the last build

@@ -1,7 +1,7 @@
 // finish.hip -- per-group completion of a force evaluation and the rigid
 // water velocity-Verlet step.
 //
-// Per sorted group (one thread):
+// Per group (one thread, groups in site order):
 //   * real-space pair results + k-space/P3M field: F = q E + F_LJ, phi
 //     (ElecLJ::add_to_energy_and_forces, eleclj.cpp:794-805);
 //   * excluded intramolecular pairs: -erf(eta r)/r with minimum image
@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
     int N, int want_virial, double bond, double angle_rad, double tol,
     double *__restrict__ part, int *__restrict__ done, int maxblk, double *__restrict__ acc,
     int *__restrict__ rebuild, int *__restrict__ err, double4 *__restrict__ gpos,
-    float4 *__restrict__ spk, const int *__restrict__ ginfo, const int4 *__restrict__ gslot) {
+    float4 *__restrict__ spk, const int *__restrict__ ginfo, const int *__restrict__ gslot_inv) {
   constexpr int NPART = FinSlots<VIR>::N, TOP = FinSlots<VIR>::TOP;
   __shared__ double red[FIN_TB / 64][NPART];
   __shared__ int islast;
@@ -56,10 +56,12 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
 #pragma unroll
   for (int c = 0; c < NPART; c++) vals[c] = 0;
   if (active) {
-    const int gs = P.group0 + i;
-    const int4 sl = gslot[gs];   // group, first site, site count, exclusion mask
-    const int g = sl.x;
-    const int s0 = sl.y, ns = sl.z;
+    // threads take groups in site order, so the site arrays (positions,
+    // velocities, forces, k-space results) are read and written coalesced;
+    // the pair results and packed inputs sit at the group's slot
+    const int g = P.group0 + i;
+    const int gs = gslot_inv[g];
+    const int s0 = gfirst[g], ns = gns[g];
     constexpr int K = 3;
     double F[K][3], ph[K], qq[K], x[K][3];
 #pragma unroll
@@ -79,7 +81,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
       }
     }
     // excluded pairs inside the group (rspace.c:492-521)
-    const int ex = sl.w;
+    const int ex = gexcl[g];
     if (ex) {
       const double L = P.L, invL = P.invL, eta = P.eta;
 #pragma unroll
@@ -298,7 +300,7 @@ int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out) {
 #define FIN_ARGS e.d_sp, e.d_order, e.d_gfirst, e.d_gns, e.d_gexcl, e.d_q, e.d_mass, e.d_pf, \
     e.d_gacc, e.d_kres, e.d_r, e.d_v, e.d_f, phi_out, e.d_last, e.nsite, want_virial, e.bond,  \
     e.angle * (M_PI / 180.0), e.tol, e.d_fpart, e.d_done, e.nblk_part, e.d_acc, e.d_rebuild,  \
-    e.d_err, e.d_gpos, e.d_spk, e.d_ginfo, e.d_gslot
+    e.d_err, e.d_gpos, e.d_spk, e.d_ginfo, e.d_gslot_inv
 #define FIN_CASE(M)                                                                            \
   if (want_virial)                                                                             \
     hipLaunchKernelGGL((k_finish<M, true, false>), grid, dim3(FIN_TB), 0, e.stream, FIN_ARGS);  \

@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
     const int *__restrict__ gfirst, const int *__restrict__ gns, const int *__restrict__ ginfo,
     const double4 *__restrict__ gpos, float4 *__restrict__ spk, int2 *__restrict__ gmeta,
     const int *__restrict__ gexcl, int4 *__restrict__ gslot, const int *__restrict__ cell_rank,
-    const int *__restrict__ cell_unrank) {
+    const int *__restrict__ cell_unrank, int *__restrict__ gslot_inv) {
 #pragma clang fp contract(off)
   __shared__ int c_start[4][NL_SEG], c_off[4][NL_SEG + 1], c_cell[4][NL_SEG];
   const int s = blockIdx.y;
@@ -199,7 +199,10 @@ __global__ void __launch_bounds__(256) k_nl_pairs(
     const int gs = P.group0 + wave;
     const int g = order[gs];
     // the finish kernel's per-slot indices in the new order
-    if (lane == 0) gslot[gs] = make_int4(g, gfirst[g], gns[g], gexcl[g]);
+    if (lane == 0) {
+      gslot[gs] = make_int4(g, gfirst[g], gns[g], gexcl[g]);
+      gslot_inv[g] = gs;
+    }
     if (bad) {   // the error is reported; the rows stay empty
       if (lane == 0) nnb[gs] = 0;
       continue;
@@ -399,7 +402,7 @@ int launch_nl_build(Engine &e, int force, int from_partials) {
                      e.stream, e.d_sp, e.d_rebuild, e.d_order, e.d_gcell, e.d_cell_start,
                      e.d_sten, e.nbcap, e.d_nbr, e.d_nnb, e.d_err, e.d_r, e.nsite, e.d_q,
                      e.d_gfirst, e.d_gns, e.d_ginfo, e.d_gpos, e.d_spk, e.d_gmeta, e.d_gexcl,
-                     e.d_gslot, e.d_cell_rank, e.d_cell_unrank);
+                     e.d_gslot, e.d_cell_rank, e.d_cell_unrank, e.d_gslot_inv);
   MM_HIP(hipGetLastError());
   return 0;
 }
