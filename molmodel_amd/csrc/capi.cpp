@@ -352,9 +352,10 @@ static int restore_state(Engine &e, int remd) {
 }
 
 // nl_from_partials: the last finish left per-block partial sums (FIN_LAZY);
-// mesh_energy 0: an inner MD step, whose energies are not formed
+// energy 0: an inner MD step, whose energies nobody reads: the pair kernel
+// and the P3M x pass form forces only
 static int enqueue_forces(mm_engine *m, int field_mode, int want_virial,
-                          int nl_from_partials = 0, int mesh_energy = 1) {
+                          int nl_from_partials = 0, int energy = 1) {
   Engine &e = m->e;
   if (e.no_rspace) {
     TRY(launch_kspace(e, want_virial));
@@ -373,7 +374,7 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial,
     kst = e.stream2;
   }
   TRY(launch_kspace(e, want_virial, kst));
-  TRY(launch_p3m(e, want_virial, kst, mesh_energy || want_virial));
+  TRY(launch_p3m(e, want_virial, kst, energy || want_virial));
   if (e.external_nl) {
     e.first_eval = 0;
   } else if (e.first_eval) {
@@ -386,7 +387,7 @@ static int enqueue_forces(mm_engine *m, int field_mode, int want_virial,
   if (e.need_pack || m->pos_dirty) TRY(launch_pack(e));
   e.need_pack = 0;
   m->pos_dirty = 0;
-  TRY(launch_pair(e, field_mode, want_virial));
+  TRY(launch_pair(e, field_mode, want_virial, energy || field_mode || want_virial));
   if (fork) {
     MM_HIP(hipEventRecord(e.ev_join, e.stream2));
     MM_HIP(hipStreamWaitEvent(e.stream, e.ev_join, 0));

@@ -222,7 +222,7 @@ int engine_grow_bins(Engine &e);
 // displacement partials of the last (FIN_LAZY) finish exceed the skin
 int launch_nl_build(Engine &e, int force, int from_partials = 0);
 int launch_pack(Engine &e);
-int launch_pair(Engine &e, int field_mode, int want_virial);
+int launch_pair(Engine &e, int field_mode, int want_virial, int energy = 1);
 int pair_tiles(int ngroup);
 // mesh lines per FFT workgroup (16 threads per line); the x pass writes one
 // partial-sum slot per workgroup

@@ -76,7 +76,8 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
         F[k][0] = o[0] + qq[k] * kr[1];
         F[k][1] = o[1] + qq[k] * kr[2];
         F[k][2] = o[2] + qq[k] * kr[3];
-        ph[k] = o[3] + kr[0] - P.self_coef * qq[k] + P.bg_phi;
+        // lazy (inner MD) steps: the pair kernel formed forces only
+        if (!LAZY) ph[k] = o[3] + kr[0] - P.self_coef * qq[k] + P.bg_phi;
         x[k][0] = r[sk]; x[k][1] = r[N + sk]; x[k][2] = r[2 * N + sk];
       }
     }
@@ -109,10 +110,17 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
           }
         }
     }
-#pragma unroll
-    for (int k = 0; k < K; k++) vals[0] += 0.5 * qq[k] * ph[k];
     const double *ga = gacc + (long)gs * 10;
-    vals[1] = ga[0];
+    if constexpr (LAZY) {
+      // no energies: slot 0 carries the force sum, so a non-finite force
+      // still raises ERR_NAN below
+#pragma unroll
+      for (int k = 0; k < K; k++) vals[0] += F[k][0] + F[k][1] + F[k][2];
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; k++) vals[0] += 0.5 * qq[k] * ph[k];
+      vals[1] = ga[0];
+    }
     if constexpr (VIR) {
 #pragma unroll
       for (int c = 0; c < 9; c++) vals[3 + c] += ga[1 + c];
@@ -122,7 +130,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
       if (k < ns) {
         const int sk = s0 + k;
         f[sk] = F[k][0]; f[N + sk] = F[k][1]; f[2 * N + sk] = F[k][2];
-        if (phi_out) phi_out[sk] = ph[k];
+        if (!LAZY && phi_out) phi_out[sk] = ph[k];
       }
     if (MODE) {
       const double dt = P.dt;

@@ -30,7 +30,7 @@ enum : int {
   ERR_NL_OVERFLOW = 1,       // a group has more neighbors than the list holds
   ERR_TABLE_RANGE = 2,       // a site pair fell outside the erfc table (exact path used)
   ERR_CONSTRAINT = 4,        // SHAKE did not converge
-  ERR_NAN = 8,               // non-finite energy
+  ERR_NAN = 8,               // non-finite energy (or force, in inner MD steps)
   ERR_BIN_OVERFLOW = 16,     // P3M column bin overflow
   ERR_CELL = 32,             // a neighbour-list cell holds more groups than its sort takes
 };

@@ -132,7 +132,7 @@ __device__ __forceinline__ void coul_table(const TabView &T, const float4 *stab,
 
 // One tile: the GPB i-groups blk * GPB ... of system P.  The block's LDS
 // holds that system's erfc and LJ tables.
-template <bool FIELD, bool VIRIAL, bool WATER>
+template <bool FIELD, bool VIRIAL, bool WATER, bool ENERGY>
 __device__ __forceinline__ void pair_tile(
     const SysParams &P, int blk, const float4 *stab, const float2 *slj, float4 (*qd)[QCAP],
     float2 (*qs)[QCAP], const double4 *__restrict__ gpos, const float4 *__restrict__ spk,
@@ -286,7 +286,8 @@ __device__ __forceinline__ void pair_tile(
         // unsmoothed pair energy over the charges, then the smoothed potentials
         tmps = oi[0].w * ph[0] + oi[1].w * ph[1] + oi[2].w * ph[2];
 #pragma unroll
-        for (int k = 0; k < 3; k++) ph[k] *= smf;
+        for (int k = 0; k < 3; k++)
+          if (ENERGY) ph[k] *= smf;
         general = worst >= T.ntab;
       }
       if (general) {
@@ -313,7 +314,7 @@ __device__ __forceinline__ void pair_tile(
             float b0, e;
             coul_pair(T, stab, r2, qs, qsA, qsB, b0, e);
             tq += oi[k].w * b0;
-            ph[k] += qs * b0;
+            if (ENERGY) ph[k] += qs * b0;
             fx[k] += e * rx; fy[k] += e * ry; fz[k] += e * rz;
           }
           if (m & SITE_HAS_LJ) {
@@ -331,14 +332,14 @@ __device__ __forceinline__ void pair_tile(
       }
       }
       tmps += uu;
-      uu *= smf;
+      if (ENERGY) uu *= smf;
       // smoothing force on the first site (rspace.c:479-489)
       const float fs = dsmf * tmps;
       const float sx = fs * Dx, sy = fs * Dy, sz = fs * Dz;
       double Fx = 0, Fy = 0, Fz = 0;
 #pragma unroll
       for (int k = 0; k < NSI; k++) {
-        ap[k] += (double)ph[k];
+        if (ENERGY) ap[k] += (double)ph[k];
         if (FIELD) {
           ax[k] += (double)fx[k]; ay[k] += (double)fy[k]; az[k] += (double)fz[k];
           lx[k] += (double)gx[k]; ly[k] += (double)gy[k]; lz[k] += (double)gz[k];
@@ -352,7 +353,7 @@ __device__ __forceinline__ void pair_tile(
         }
       }
       if (FIELD) { lx[0] += (double)sx; ly[0] += (double)sy; lz[0] += (double)sz; }
-      ulj += (double)uu;
+      if (ENERGY) ulj += (double)uu;
       if (VIRIAL) {   // 1/2 D (x) F_gh per directed pair
         const double dx = Dx, dy = Dy, dz = Dz;
         w[0] += 0.5 * dx * Fx; w[1] += 0.5 * dx * Fy; w[2] += 0.5 * dx * Fz;
@@ -369,13 +370,14 @@ __device__ __forceinline__ void pair_tile(
 #pragma unroll
     for (int k = 0; k < NSI; k++) {
       ax[k] += shfl_xor_d(ax[k], m); ay[k] += shfl_xor_d(ay[k], m);
-      az[k] += shfl_xor_d(az[k], m); ap[k] += shfl_xor_d(ap[k], m);
+      az[k] += shfl_xor_d(az[k], m);
+      if (ENERGY) ap[k] += shfl_xor_d(ap[k], m);
       if (FIELD) {
         lx[k] += shfl_xor_d(lx[k], m); ly[k] += shfl_xor_d(ly[k], m);
         lz[k] += shfl_xor_d(lz[k], m);
       }
     }
-    ulj += shfl_xor_d(ulj, m);
+    if (ENERGY) ulj += shfl_xor_d(ulj, m);
     if (VIRIAL) {
 #pragma unroll
       for (int c = 0; c < 9; c++) w[c] += shfl_xor_d(w[c], m);
@@ -385,14 +387,15 @@ __device__ __forceinline__ void pair_tile(
 #pragma unroll
   for (int k = 0; k < NSI; k++) {
     double *o = pf + ((long)gs * MAXS + k) * 4;
-    o[0] = ax[k]; o[1] = ay[k]; o[2] = az[k]; o[3] = ap[k];
+    o[0] = ax[k]; o[1] = ay[k]; o[2] = az[k];
+    if (ENERGY) o[3] = ap[k];
     if (FIELD) {
       double *l = plj + ((long)gs * MAXS + k) * 3;
       l[0] = lx[k]; l[1] = ly[k]; l[2] = lz[k];
     }
   }
   double *ga = gacc + (long)gs * 10;
-  ga[0] = 0.5 * ulj;   // each unordered pair is visited from both sides
+  if (ENERGY) ga[0] = 0.5 * ulj;   // each unordered pair is visited from both sides
   if (VIRIAL) {
     // per-site offset term: sum_k o_k (x) F_k (o_0 = 0)
 #pragma unroll
@@ -421,7 +424,7 @@ __device__ __forceinline__ void pair_tile(
 // another system's tables.  The host sizes the grid below the device's
 // resident capacity, which leaves room on every CU for the reciprocal-space
 // kernels running on the other stream.
-template <bool FIELD, bool VIRIAL, bool WATER>
+template <bool FIELD, bool VIRIAL, bool WATER, bool ENERGY>
 __global__ void __launch_bounds__(PAIR_TB, (FIELD || VIRIAL) ? 1 : 4) k_pair(
     const SysParams *__restrict__ sp, const int *__restrict__ tile_start, int nsys, int ntile,
     const double4 *__restrict__ gpos, const float4 *__restrict__ spk,
@@ -451,7 +454,7 @@ __global__ void __launch_bounds__(PAIR_TB, (FIELD || VIRIAL) ? 1 : 4) k_pair(
       __syncthreads();
       cur = a;
     }
-    pair_tile<FIELD, VIRIAL, WATER>(P, t - tile_start[a], stab, slj, qd, qs, gpos, spk, gmeta, nbr,
+    pair_tile<FIELD, VIRIAL, WATER, ENERGY>(P, t - tile_start[a], stab, slj, qd, qs, gpos, spk, gmeta, nbr,
                              nnb, nbcap, pf, plj, gacc);
   }
 }
@@ -757,7 +760,7 @@ int launch_nonperiodic_points(const double4 *site, const double *rscreen, int n,
 
 int pair_tiles(int ngroup) { return (ngroup + GPB - 1) / GPB; }
 
-int launch_pair(Engine &e, int field_mode, int want_virial) {
+int launch_pair(Engine &e, int field_mode, int want_virial, int energy) {
   if (e.ntile == 0) return 0;
   dim3 grid(e.pair_blocks), tb(PAIR_TB);
 #define PAIR_ARGS                                                                             \
@@ -765,17 +768,20 @@ int launch_pair(Engine &e, int field_mode, int want_virial) {
       e.nbcap, e.d_tab, e.d_ljt, e.d_pf, e.d_plj, e.d_gacc
   // WATER: the kernel carries the path for systems made only of three-site
   // waters
-#define PAIR_LAUNCH(F, V)                                                                    \
+  // ENERGY false (forces only) for steps whose energies nobody reads
+#define PAIR_LAUNCH(F, V, EN)                                                                \
   if (e.any_water)                                                                           \
-    hipLaunchKernelGGL((k_pair<F, V, true>), grid, tb, 0, e.stream, PAIR_ARGS);              \
+    hipLaunchKernelGGL((k_pair<F, V, true, EN>), grid, tb, 0, e.stream, PAIR_ARGS);          \
   else                                                                                       \
-    hipLaunchKernelGGL((k_pair<F, V, false>), grid, tb, 0, e.stream, PAIR_ARGS);
+    hipLaunchKernelGGL((k_pair<F, V, false, EN>), grid, tb, 0, e.stream, PAIR_ARGS);
   if (field_mode) {
-    PAIR_LAUNCH(true, false)
+    PAIR_LAUNCH(true, false, true)
   } else if (want_virial) {
-    PAIR_LAUNCH(false, true)
+    PAIR_LAUNCH(false, true, true)
+  } else if (energy) {
+    PAIR_LAUNCH(false, false, true)
   } else {
-    PAIR_LAUNCH(false, false)
+    PAIR_LAUNCH(false, false, false)
   }
 #undef PAIR_LAUNCH
 #undef PAIR_ARGS
