@@ -125,6 +125,76 @@ int mm_derive_sites(int natom, const int *nbr_start, const int *nbr, int *atom_o
  * violation (or the input error engine creation would report). */
 int mm_check_layout(int nsys, const mm_system_desc *sys);
 
+/* ---- msim input and restart formats (host code: no GPU needed) ---- */
+
+/* free memory the library returned (mm_msim_preprocess) */
+void mm_free(void *p);
+
+/* msim's input preprocessor (PreprocessedStream, prepro.cpp:23-318).
+ * args as msim's command line after the program name: "-name value" defines
+ * a variable, any other entry is a file read in turn (and sets $base to its
+ * name without the extension); $pwd and $pid are predefined, and
+ * $MSIM_DIR/param is searched as msim does.  Per line: '#' starts a
+ * comment; $name is replaced (defined variables, then the environment);
+ * !define name value, !include file or <file, !path dir (the directory is
+ * taken without surrounding whitespace), !foreach v { values } ... !end,
+ * !repeat n ... !end, !ignore ... !end.  !parallel_define (MPI only) and
+ * reading stdin are not provided.  *out: the processed text (free with
+ * mm_free). */
+int mm_msim_preprocess(int nargs, const char *const *args, char **out);
+
+/* ElecLJ's io parameters (eleclj.hpp:310-327) read from msim text, starting
+ * from the constructor's defaults and table entries (eleclj.cpp:12-40).
+ * deck: a whole program ("{ ... potential { eleclj { ... } } ... }", every
+ * eleclj block read in turn, a bare "eleclj" meaning the defaults) or one
+ * block "{ field value ... }".  Unknown fields and malformed values are
+ * errors, as in the reference's generated reader (io:219-312). */
+typedef struct mm_msim_eleclj mm_msim_eleclj;
+int mm_msim_eleclj_read(const char *deck, mm_msim_eleclj **out);
+void mm_msim_eleclj_free(mm_msim_eleclj *p);
+/* a scalar field (numbers, integers, Booleans as 0/1; for the virtual-site,
+ * bond-charge-increment and 1-3 tables: whether any entry was given) */
+int mm_msim_eleclj_get(const mm_msim_eleclj *p, const char *field, double *value);
+/* an entry of fixed_charge, screening_radius, sigma or epsilon: 0, or 1 if
+ * the table has no such key */
+int mm_msim_eleclj_table(const mm_msim_eleclj *p, const char *table, const char *key,
+                         double *value);
+/* the descriptor's parameters from the block (d->boxl set by the caller):
+ * rspace_cutoff, smoothing_width, neighbor_list_width, ewald_screening,
+ * kspace_cutoff, use_p3m, p3m_assignment_order, p3m_alias_sum_limit,
+ * include_lj_correction, geometric_combining_for_sigma, then
+ * mm_desc_resolve with p3m_grid_spacing / p3m_grid_density.  Parameters
+ * the device path does not evaluate (virtual sites, bond charge
+ * increments, 1-3 interactions, 1-4 scaling, ik differentiation, a term
+ * switched off) are errors.  Sites and groups are left to the caller. */
+int mm_msim_eleclj_desc(const mm_msim_eleclj *p, mm_system_desc *d);
+/* per-atom charge (sqrt(kcal A/mol)), sigma, epsilon and site mask from the
+ * atoms' types as ElecLJ::types_changed derives them (eleclj.cpp:440-500):
+ * fixed_charge[type] unless almost zero -> MM_SITE_HAS_Q; sigma and
+ * epsilon of general_type[type] (or the type) when both exist and epsilon
+ * is not almost zero -> MM_SITE_HAS_LJ.  A charged type with a screening
+ * radius is an error (the device evaluates unscreened charges). */
+int mm_msim_site_params(const mm_msim_eleclj *p, int natom, const char *const *types, double *q,
+                        double *sigma, double *epsilon, int *mask);
+
+/* xyz coordinate and restart files (msys.cpp:823-868, 916-932).
+ * mm_xyz_read with natom <= 0 returns the file's atom count (> 0).  Else
+ * natom must match; per atom "symbol x y z [type [vx vy vz]]": symbols and
+ * types (may be NULL) receive 16-character slots (type "*" when absent),
+ * r[3*natom] and v[3*natom] (may be NULL) the vectors, *has_velocities
+ * whether every line held them; bc (bc_len characters, may be NULL) the
+ * boundary conditions of line 2 as "cubic L" / "orthorhombic a b c" /
+ * ..., or "" when line 2 gives none.
+ * mm_xyz_write writes the same format: "natom", "boundary_conditions bc
+ * [  comment]", then "symbol  x y z type  vx vy vz" per atom (types may be
+ * NULL or hold "" for "*", v NULL for zero velocities), numbers with
+ * `precision` significant digits (0: 6, the reference's stream default). */
+int mm_xyz_read(const char *fname, int natom, char *symbols, char *types, double *r, double *v,
+                int *has_velocities, char *bc, int bc_len);
+int mm_xyz_write(const char *fname, int natom, const char *symbols, const char *types,
+                 const double *r, const double *v, const char *bc, const char *comment,
+                 int precision);
+
 typedef struct mm_engine mm_engine;
 
 /* energy slots returned per system by mm_get_energies */

@@ -80,6 +80,18 @@ def lib():
         L.mm_desc_init.argtypes = [_P]
         L.mm_desc_resolve.argtypes = [_P, _D, _D]
         L.mm_derive_sites.argtypes = [_I, _P, _P, _P, _P, _P, _P, _L, _P, _P]
+        L.mm_free.restype = None
+        L.mm_free.argtypes = [_P]
+        L.mm_msim_preprocess.argtypes = [_I, _P, C.POINTER(_P)]
+        L.mm_msim_eleclj_read.argtypes = [C.c_char_p, C.POINTER(_P)]
+        L.mm_msim_eleclj_free.restype = None
+        L.mm_msim_eleclj_free.argtypes = [_P]
+        L.mm_msim_eleclj_get.argtypes = [_P, C.c_char_p, _P]
+        L.mm_msim_eleclj_table.argtypes = [_P, C.c_char_p, C.c_char_p, _P]
+        L.mm_msim_eleclj_desc.argtypes = [_P, _P]
+        L.mm_msim_site_params.argtypes = [_P, _I, _P, _P, _P, _P, _P]
+        L.mm_xyz_read.argtypes = [C.c_char_p, _I, _P, _P, _P, _P, _P, C.c_char_p, _I]
+        L.mm_xyz_write.argtypes = [C.c_char_p, _I, _P, _P, _P, _P, C.c_char_p, C.c_char_p, _I]
         _lib = L
     return _lib
 
@@ -158,6 +170,88 @@ def derive_sites(natom, bonds_of):
     _check(lib().mm_derive_sites(*args(excl, nexcl.value)))
     return (atom_of_site, group[:2 * ngroup.value].reshape(-1, 2),
             excl[:2 * nexcl.value].reshape(-1, 2), ncross.value)
+
+
+def msim_preprocess(args):
+    """msim's input preprocessor over the files (and -name value pairs) of
+    an msim command line (mm_msim_preprocess); returns the processed text."""
+    arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
+    out = _P()
+    _check(lib().mm_msim_preprocess(len(args), C.cast(arr, _P), C.byref(out)))
+    try:
+        return C.string_at(out.value).decode()
+    finally:
+        lib().mm_free(out)
+
+
+class MsimElecLJ:
+    """ElecLJ's io parameters read from msim text (mm_msim_eleclj_read)."""
+
+    def __init__(self, deck):
+        self.h = _P()
+        _check(lib().mm_msim_eleclj_read(deck.encode(), C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mm_msim_eleclj_free(self.h)
+            self.h = None
+
+    def __getitem__(self, field):
+        v = C.c_double()
+        _check(lib().mm_msim_eleclj_get(self.h, field.encode(), C.byref(v)))
+        return v.value
+
+    def table(self, table, key):
+        v = C.c_double()
+        rc = _check(lib().mm_msim_eleclj_table(self.h, table.encode(), key.encode(), C.byref(v)))
+        return None if rc == 1 else v.value
+
+    def desc(self, boxl):
+        """(rc, smoothing_width, skin, eta, kmode, kc, mesh, order, nalias,
+        include_lj_correction, geometric_sigma) after mm_msim_eleclj_desc."""
+        d = SystemDesc()
+        d.boxl = boxl
+        _check(lib().mm_msim_eleclj_desc(self.h, C.byref(d)))
+        return dict(rc=d.rc, smoothing_width=d.smoothing_width, skin=d.skin, eta=d.eta,
+                    kmode=d.kmode, kc=d.kc, mesh=tuple(d.mesh), order=d.order, nalias=d.nalias,
+                    include_lj_correction=d.include_lj_correction,
+                    geometric_sigma=d.geometric_sigma)
+
+    def site_params(self, types):
+        n = len(types)
+        arr = (C.c_char_p * max(n, 1))(*[t.encode() for t in types])
+        q, sig, eps = (np.zeros(n) for _ in range(3))
+        mask = np.zeros(n, dtype=np.int32)
+        _check(lib().mm_msim_site_params(self.h, n, C.cast(arr, _P), _ptr(q), _ptr(sig),
+                                         _ptr(eps), _ptr(mask)))
+        return q, sig, eps, mask
+
+
+def xyz_read(fname):
+    """An xyz coordinate / restart file (mm_xyz_read): dict of symbols,
+    types, r [n, 3], v [n, 3] or None, bc (str)."""
+    n = _check(lib().mm_xyz_read(fname.encode(), 0, None, None, None, None, None, None, 0))
+    sym = C.create_string_buffer(16 * n)
+    typ = C.create_string_buffer(16 * n)
+    r, v = np.zeros((n, 3)), np.zeros((n, 3))
+    hv = C.c_int(0)
+    bc = C.create_string_buffer(256)
+    _check(lib().mm_xyz_read(fname.encode(), n, sym, typ, _ptr(r), _ptr(v), C.byref(hv), bc, 256))
+    cut = lambda b, i: b.raw[16 * i:16 * i + 16].split(b"\0")[0].decode()
+    return dict(symbols=[cut(sym, i) for i in range(n)], types=[cut(typ, i) for i in range(n)],
+                r=r, v=v if hv.value else None, bc=bc.value.decode())
+
+
+def xyz_write(fname, symbols, r, bc, types=None, v=None, comment="", precision=0):
+    """Write an xyz coordinate / restart file (mm_xyz_write)."""
+    n = len(symbols)
+    slot = lambda names: b"".join(x.encode()[:15].ljust(16, b"\0") for x in names)
+    sym = C.create_string_buffer(slot(symbols), 16 * max(n, 1))
+    typ = C.create_string_buffer(slot(types), 16 * max(n, 1)) if types is not None else None
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    vv = np.ascontiguousarray(v, dtype=np.float64) if v is not None else None
+    _check(lib().mm_xyz_write(fname.encode(), n, sym, typ, _ptr(r), _ptr(vv) if vv is not None else None,
+                              bc.encode(), comment.encode(), precision))
 
 
 def _check(rc):

@@ -1,0 +1,99 @@
+"""An msim-style deck and an xyz restart driving the engine end to end.
+
+The deck holds ElecLJ's parameters in msim's object syntax, split over two
+files joined by the preprocessor (an include, a !define and a command-line
+variable); the restart holds the 648-atom water box in the reference's xyz
+format at full precision.  From them: the descriptor (mm_msim_eleclj_desc),
+per-atom parameters from the atom types (mm_msim_site_params), sites,
+groups and exclusions from the molecules' bonds (mm_derive_sites).  The
+engine built this way must give bit for bit the forces and energies of the
+engine built from the generator's own arrays, and match the FP64 oracle
+to the parity bar (force relative RMS <= 1e-5).
+"""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from molmodel_amd import engine as E
+from molmodel_amd import waterbox as W
+
+pytestmark = pytest.mark.gpu
+
+POTENTIAL = """eleclj {
+  rspace_cutoff $cutoff
+  smoothing_width $width
+  neighbor_list_width %r
+  kspace_cutoff 7  ewald_screening 0.35
+  fixed_charge { OW -0.8476 HW 0.4238 }
+  sigma { OW 3.166 }  epsilon { OW 0.1553 HW 0 }
+}
+"""
+
+DECK = """# water box, Ewald
+!define width 1
+{
+  molsys { }
+  potential {
+    <%s
+  }
+  dynamics { n 10 }
+}
+"""
+
+
+def test_deck_and_restart_drive_the_engine(gpu_lib, tmp_path):
+    s, p = W.config("water648_ewald")
+    d = str(tmp_path)
+    n = s.nsite
+    print(f"[msim] water648 from a deck, N={n}", flush=True)
+    sym = ["O", "H", "H"] * (n // 3)
+    typ = ["OW", "HW", "HW"] * (n // 3)
+    xyz = os.path.join(d, "start.xyz")
+    E.xyz_write(xyz, sym, s.r, f"cubic {s.boxl!r}", types=typ, v=s.v, precision=17,
+                comment="Time elapsed: 0.000000 psec")
+    pot = os.path.join(d, "pot.inp")
+    with open(pot, "w") as f:
+        f.write(POTENTIAL % p["skin"])
+    deck = os.path.join(d, "water.inp")
+    with open(deck, "w") as f:
+        f.write(DECK % pot)
+
+    m = E.MsimElecLJ(E.msim_preprocess(["-cutoff", "9", deck]))
+    x = E.xyz_read(xyz)
+    kind, boxl = x["bc"].split()
+    assert kind == "cubic"
+    prm = m.desc(float(boxl))
+    q, sig, eps, mask = m.site_params(x["types"])
+    bonds = [[i + 1, i + 2] if i % 3 == 0 else [i - i % 3] for i in range(n)]
+    atom_of_site, groups, excl, ncross = E.derive_sites(n, bonds)
+    assert ncross == 0 and np.array_equal(atom_of_site, np.arange(n))
+
+    params = dict(p, rc=prm["rc"], smoothing_width=prm["smoothing_width"], eta=prm["eta"],
+                  kmode=prm["kmode"], kc=prm["kc"], skin=prm["skin"], nalias=prm["nalias"],
+                  include_lj_correction=prm["include_lj_correction"])
+    s2 = dataclasses.replace(s, r=x["r"], v=x["v"], q=q, sig=sig, eps=eps, site_mask=mask,
+                             group=groups.astype(np.int32), excl=excl.astype(np.int32))
+    # the deck and the restart reproduce the generator's inputs exactly
+    assert params == p
+    for a, b in ((s2.r, s.r), (s2.v, s.v), (s2.q, s.q), (s2.sig, s.sig), (s2.eps, s.eps),
+                 (s2.site_mask, s.site_mask), (s2.group, s.group)):
+        assert np.array_equal(a, b)
+    # the same excluded pairs (mm_derive_sites lists them sorted)
+    assert np.array_equal(np.unique(s2.excl, axis=0), np.unique(s.excl, axis=0))
+
+    e1, e0 = E.Engine(s2, params), E.Engine(s, p)
+    for e, sys_ in ((e1, s2), (e0, s)):
+        e.set_positions(sys_.r)
+        e.compute()
+    f1, f0 = e1.forces(), e0.forces()
+    en1, en0 = e1.energies()[0], e0.energies()[0]
+    assert np.array_equal(f1, f0) and np.array_equal(en1, en0)
+    of, oen = O.evaluate(s2, params)
+    ferr = float(np.linalg.norm(f1 - of) / np.linalg.norm(of))
+    print(f"[msim] force rel {ferr:.3e}, elec {en1[0]:.8f} vs {oen[0]:.8f}", flush=True)
+    assert ferr <= 1e-5
+    assert abs(en1[0] - oen[0]) <= 1e-5 * abs(oen[3])
+    assert abs(en1[1] - oen[1]) <= 1e-5 * abs(oen[3])

@@ -1,0 +1,302 @@
+"""msim input and restart formats (csrc/msimio.cpp), on the CPU.
+
+Expected outputs are worked by hand from the reference's code:
+  * the preprocessor, prepro.cpp:120-318: a !define line leaves the text
+    before the directive plus a newline (so an empty line); its value starts
+    one character after the name and loses trailing whitespace; an include
+    emits the text before it, one space, the file, then the rest of the line;
+    !end after !ignore blanks the directive and keeps the rest of the line;
+  * ElecLJ's generated reader (io:219-312), the constructor's defaults and
+    table entries (eleclj.cpp:12-40), Boolean's words (boo.hpp:26-41);
+  * per-type site parameters (eleclj.cpp:440-500);
+  * xyz files (msys.cpp:823-868, 916-932; atom.cpp:129-134), numbers at the
+    stream default of 6 significant digits.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from molmodel_amd import engine as E
+from molmodel_amd import waterbox as W
+
+
+def write(d, name, text):
+    p = os.path.join(d, name)
+    os.makedirs(os.path.dirname(p), exist_ok=True)
+    with open(p, "w") as f:
+        f.write(text)
+    return p
+
+
+def pre(d, text, extra=()):
+    return E.msim_preprocess(list(extra) + [write(d, "deck.inp", text)])
+
+
+# ------------------------------------------------------------ preprocessor
+
+def test_comments_defines_and_substitution(tmp_path):
+    d = str(tmp_path)
+    text = ("!define rc 9.5   # cutoff\n"
+            "x $rc y\n"
+            "!define  w  1 2  \n"
+            "a $w b   # trailing comment\n"
+            "keep this # but not $this\n")
+    assert pre(d, text) == "\nx 9.5 y\n\na  1 2 b   \nkeep this \n"
+
+
+def test_environment_and_command_line_variables(tmp_path, monkeypatch):
+    d = str(tmp_path)
+    monkeypatch.setenv("MMTEST_X", "7")
+    monkeypatch.setenv("MMTEST_Y", "env")
+    text = "$MMTEST_X $n\n!define MMTEST_Y def\n$MMTEST_Y $base\n"
+    out = pre(d, text, extra=["-n", "3"])
+    assert out == f"7 3\n\ndef {os.path.join(d, 'deck')}\n"
+
+
+def test_substitution_is_processed_again(tmp_path):
+    d = str(tmp_path)
+    text = "!define a $b\n!define b 42\n!define c v$a\n[$c]\n"
+    # $a is replaced when !define a is read: b is undefined at that point
+    with pytest.raises(E.MMError, match="undefined variable: 'b'"):
+        pre(d, text)
+    text = "!define b 42\n!define a $b\n!define c v$a\n[$c]\n"
+    assert pre(d, text) == "\n\n\n[v42]\n"
+
+
+def test_include_forms(tmp_path):
+    d = str(tmp_path)
+    write(d, "inc.txt", "X\n")
+    inc = os.path.join(d, "inc.txt")
+    text = f"a\n!include {inc} b\nc <{inc} d\n\"not <an include\" e\n"
+    assert pre(d, text) == "a\n X\n b\nc  X\n d\n\"not <an include\" e\n"
+
+
+def test_path_directive(tmp_path):
+    d = str(tmp_path)
+    write(d, "sub/f.txt", "from sub\n")
+    text = f"!path   {os.path.join(d, 'sub')}   \n<f.txt\n"
+    assert pre(d, text) == "\n from sub\n\n"
+
+
+def test_foreach_repeat_and_nesting(tmp_path):
+    d = str(tmp_path)
+    text = ("!foreach v { a b c }\n"
+            "item $v\n"
+            "!end\n"
+            "!repeat 2\n"
+            "r\n"
+            "!foreach w x\n"
+            "  $w\n"
+            "!end\n"
+            "!end\n"
+            "done\n")
+    assert pre(d, text) == "item a\nitem b\nitem c\nr\n  x\nr\n  x\ndone\n"
+
+
+def test_ignore_block(tmp_path):
+    d = str(tmp_path)
+    text = "a\n!ignore\nhidden $undefined\n  !end shown\nafter\n"
+    assert pre(d, text) == "a\n       shown\nafter\n"
+
+
+@pytest.mark.parametrize("text, msg", [
+    ("x $nosuchvariable_q\n", "undefined variable: 'nosuchvariable_q'"),
+    ("x $ y\n", "no variable name after '\\$'"),
+    ("!end\n", "unexpected !end"),
+    ("!frobnicate\n", "unknown command"),
+    ("!foreach v { a b }\nx\n", "expecting !end"),
+    ("!foreach v { }\nx\n!end\n", "expecting !foreach"),
+    ("!repeat 0\nx\n!end\n", "expecting !repeat n where n > 0"),
+    ("<no_such_file.inp\n", "cannot open no_such_file.inp"),
+    ("!define 9-x 1\n", "illegal variable name"),
+])
+def test_preprocessor_errors(tmp_path, text, msg):
+    with pytest.raises(E.MMError, match=msg):
+        pre(str(tmp_path), text)
+
+
+# ------------------------------------------------------------ ElecLJ block
+
+def test_defaults_of_the_constructor():
+    p = E.MsimElecLJ("{ }")
+    expect = dict(rspace_cutoff=0, smoothing_width=1, neighbor_list_width=0, kspace_cutoff=5,
+                  ewald_screening=-1, tolerance=1e-10, p3m_grid_spacing=0.5,
+                  p3m_grid_density=-1, p3m_alias_sum_limit=2, p3m_assignment_order=3,
+                  elec_1_4_scale_factor=1, lj_1_4_scale_factor=1, include_lj_correction=1,
+                  include_rspace=1, include_kspace=1, include_self=1, use_p3m=0,
+                  p3m_ik_differentiate=0, geometric_combining_for_sigma=0, reorder_sites=0,
+                  verbose=1, max_iterations=1000, bond_charge_increment_frequency=1500)
+    for k, v in expect.items():
+        assert p[k] == v, k
+    assert p.table("sigma", "Lj") == 1 and p.table("epsilon", "Lj") == 1
+    assert p.table("sigma", "dummy") == 1 and p.table("epsilon", "dummy") == 0
+    assert p.table("screening_radius", "dummy") == 2 and p.table("fixed_charge", "dummy") == 0
+    assert p.table("sigma", "OW") is None
+
+
+@pytest.mark.parametrize("word, value", [("yes", 1), ("Y", 1), ("t", 1), ("TRUE", 1), ("1", 1),
+                                         ("no", 0), ("n", 0), ("F", 0), ("false", 0), ("0", 0)])
+def test_boolean_words(word, value):
+    assert E.MsimElecLJ("{ use_p3m %s }" % word)["use_p3m"] == value
+
+
+def test_fields_in_any_order_and_tables_accumulate():
+    p = E.MsimElecLJ("""{ sigma { OW 3.166 } rspace_cutoff 9
+        epsilon { OW 0.1553 HW 0 }
+        fixed_charge { OW -0.8476 HW 0.4238 } sigma { HW 1.0 } p3m_assignment_order 5 }""")
+    assert p["rspace_cutoff"] == 9 and p["p3m_assignment_order"] == 5
+    assert p.table("sigma", "OW") == 3.166 and p.table("sigma", "HW") == 1.0
+    assert p.table("sigma", "Lj") == 1.0   # the constructor's entries stay
+
+
+@pytest.mark.parametrize("text, msg", [
+    ("{ rspace_cuttoff 9 }", "unknown field name: 'rspace_cuttoff'.*'rspace_cutoff'"),
+    ("{ use_p3m maybe }", "Boolean: expecting"),
+    ("{ rspace_cutoff nine }", "expecting a number for 'rspace_cutoff'"),
+    ("{ p3m_assignment_order 4.5 }", "expecting an integer"),
+    ("{ sigma params.tab }", "read from file"),
+    ("{ rspace_cutoff 9 ", "EOF or error reading class 'ElecLJ'"),
+    ("rspace_cutoff 9", "expecting '\\{'"),
+    ("{ x { potential { } } potential { covalent } }", "has no eleclj"),
+])
+def test_reader_errors(text, msg):
+    with pytest.raises(E.MMError, match=msg):
+        E.MsimElecLJ(text)
+
+
+def test_program_deck():
+    deck = """{
+      molsys { atom { O 1 } }
+      random_seed 5
+      potential {
+        covalent
+        eleclj { rspace_cutoff 9 use_p3m yes sigma { OW 3.166 } }
+        intra { }
+        eleclj { rspace_cutoff 8.5 }
+      }
+      dynamics { n 100 }
+    }"""
+    p = E.MsimElecLJ(deck)
+    # the same ElecLJ read twice: the later value wins, the rest stays
+    assert p["rspace_cutoff"] == 8.5 and p["use_p3m"] == 1 and p.table("sigma", "OW") == 3.166
+    assert E.MsimElecLJ("{ potential { eleclj } }")["kspace_cutoff"] == 5
+
+
+def test_descriptor_matches_the_defaults_path():
+    boxl = 24.86
+    p = E.MsimElecLJ("{ kspace_cutoff 7 }")
+    d = p.desc(boxl)
+    ref = E.eleclj_defaults(boxl, kc=7.0)
+    for k in ("rc", "skin", "eta", "kc", "smoothing_width"):
+        assert d[k] == ref[k], k
+    assert d["rc"] == 0.45 * boxl and d["kmode"] == 0
+    p = E.MsimElecLJ("{ use_p3m yes rspace_cutoff 10 p3m_grid_spacing 0.8 p3m_assignment_order 5 }")
+    d = p.desc(49.7)
+    ref = E.eleclj_defaults(49.7, p3m=True, p3m_grid_spacing=0.8, rc=10.0, order=5)
+    assert d["kmode"] == 1 and d["mesh"] == tuple(ref["mesh"]) and d["eta"] == ref["eta"]
+    assert d["order"] == 5 and d["nalias"] == 2
+
+
+@pytest.mark.parametrize("text, msg", [
+    ("{ msite { (O H H) { type x r 0.1 } } }", "'msite'"),
+    ("{ one_three_interaction bci.tab }", "'one_three_interaction'"),
+    ("{ include_kspace no }", "include_kspace"),
+    ("{ elec_1_4_scale_factor 0.5 }", "1-4"),
+    ("{ p3m_ik_differentiate yes }", "ik"),
+])
+def test_descriptor_rejects_what_the_device_does_not_evaluate(text, msg):
+    with pytest.raises(E.MMError, match=msg):
+        E.MsimElecLJ(text).desc(20.0)
+
+
+def test_empty_unsupported_tables_are_fine():
+    E.MsimElecLJ("{ msite { } sp3 { } bond_charge_increment { } }").desc(20.0)
+
+
+def test_site_parameters_from_types():
+    p = E.MsimElecLJ("""{ fixed_charge { OW -0.8476 HW 0.4238 Na 1 Tiny 1e-20 }
+        sigma { OW 3.166 Na 2.35 Ow2 9 } epsilon { OW 0.1553 Na 0.13 HW 0 Ow2 0.2 }
+        general_type { OW2 OW } }""")
+    q, sig, eps, mask = p.site_params(["OW", "HW", "Na", "OW2", "Tiny", "X", "dummy", "Lj"])
+    cu = W.e_to_charge_unit(1.0)
+    assert np.array_equal(q, np.array([-0.8476, 0.4238, 1, 0, 0, 0, 0, 0]) * cu)
+    assert list(mask) == [3, 1, 3, 2, 0, 0, 0, 2]
+    assert list(sig) == [3.166, 0, 2.35, 3.166, 0, 0, 0, 1.0]
+    assert list(eps) == [0.1553, 0, 0.13, 0.1553, 0, 0, 0, 1.0]
+
+
+@pytest.mark.parametrize("text, msg", [
+    ("{ sigma { A 1 } epsilon { A -1 } }", "epsilon must be >= 0"),
+    ("{ sigma { A 0 } epsilon { A 1 } }", "sigma must be > 0"),
+    ("{ screening_radius { A 1.5 } fixed_charge { A 0.5 } }", "screening radius"),
+    ("{ screening_radius { A -1 } }", "is < 0"),
+])
+def test_site_parameter_errors(text, msg):
+    with pytest.raises(E.MMError, match=msg):
+        E.MsimElecLJ(text).site_params(["A"])
+
+
+def test_screening_of_uncharged_types_is_harmless():
+    # the constructor gives "dummy" a screening radius of 2 and no charge
+    q, sig, eps, mask = E.MsimElecLJ("{ screening_radius { B 3 } }").site_params(["dummy", "B"])
+    assert list(mask) == [0, 0] and list(q) == [0, 0]
+    with pytest.raises(E.MMError, match="screening radius"):
+        E.MsimElecLJ("{ fixed_charge { dummy 0.1 } }").site_params(["dummy"])
+
+
+# ------------------------------------------------------------ xyz files
+
+def test_xyz_reference_format(tmp_path):
+    p = str(tmp_path / "a.xyz")
+    E.xyz_write(p, ["O", "H"], [[1.23456789, -2, 3e-7], [4.5, 5, 6]], "cubic 18.64",
+                types=["OW", ""], v=[[0.1, 0.2, 0.3], [0, 0, -1e5]],
+                comment="Time elapsed: 0.000000 psec")
+    assert open(p).read() == ("2\nboundary_conditions cubic 18.64  Time elapsed: 0.000000 psec\n"
+                              "O  1.23457 -2 3e-07 OW  0.1 0.2 0.3\n"
+                              "H  4.5 5 6 *  0 0 -100000\n")
+
+
+def test_xyz_round_trip_at_full_precision(tmp_path):
+    s, _ = W.config("water648_ewald")
+    n = s.nsite
+    rng = np.random.default_rng(3)
+    r = np.asarray(s.r, dtype=float).reshape(-1, 3) if np.asarray(s.r).shape[-1] == 3 else np.asarray(s.r).T
+    v = rng.normal(size=(n, 3)) * 1e-3
+    sym = ["O", "H", "H"] * (n // 3)
+    typ = ["OW", "HW", "HW"] * (n // 3)
+    p = str(tmp_path / "w.xyz")
+    E.xyz_write(p, sym, r, f"cubic {s.boxl!r}", types=typ, v=v, precision=17)
+    x = E.xyz_read(p)
+    assert x["symbols"] == sym and x["types"] == typ
+    assert np.array_equal(x["r"], r) and np.array_equal(x["v"], v)
+    assert x["bc"] == f"cubic {s.boxl!r}"
+
+
+def test_xyz_optional_columns(tmp_path):
+    p = write(str(tmp_path), "b.xyz",
+              "3\nno boundary conditions here\nO 0 0 0\nH 1 0 0 HW\nH 0 1 0 HW 1 2 3\n")
+    x = E.xyz_read(p)
+    assert x["types"] == ["*", "HW", "HW"] and x["v"] is None and x["bc"] == ""
+    assert np.array_equal(x["r"], [[0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    p = write(str(tmp_path), "c.xyz", "1\nboundary_conditions non_periodic\nC 0 0 0 CT 0 0 0\n")
+    x = E.xyz_read(p)
+    assert x["bc"] == "non_periodic" and np.array_equal(x["v"], [[0, 0, 0]])
+    p = write(str(tmp_path), "d.xyz", "1\nboundary_conditions orthorhombic 10 11 12.5 t=0\nC 0 0 0\n")
+    assert E.xyz_read(p)["bc"] == "orthorhombic 10 11 12.5"
+
+
+@pytest.mark.parametrize("text, msg", [
+    ("x\n\n", "could not read number of atoms"),
+    ("2\nboundary_conditions cubic 10\nO 0 0 0\n", "error reading line 3"),
+    ("1\nboundary_conditions cubic 10\nO 0 0\n", "error reading line 2 of xyz"),
+    ("1\nboundary_conditions cubic 10\nO 0 0 0 OW 1 2\n", "error reading line 2 of xyz"),
+    ("1\nboundary_conditions hexagonal 10\nO 0 0 0\n", "bcondtype_t"),
+    ("1\nboundary_conditions orthorhombic 10 11\nO 0 0 0\n", "expecting 3 dimensions"),
+    ("1\nboundary_conditions cubic 10\nO 0 0 0", "error reading line 2 of xyz"),
+])
+def test_xyz_errors(tmp_path, text, msg):
+    p = write(str(tmp_path), "e.xyz", text)
+    with pytest.raises(E.MMError, match=msg):
+        E.xyz_read(p)
