@@ -56,6 +56,10 @@ def main():
                     help="run the reciprocal-space kernels on the main stream")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device copy of this many MB (counter calibration)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="run with the default shape instead of the autotuner's choice")
+    ap.add_argument("--tune-steps", type=int, default=100,
+                    help="MD steps per timed block of the autotuner (before the timed region)")
     args = ap.parse_args()
     if args.cpu_worker:   # one process of the all-core CPU sample: no GPU use
         print(json.dumps({"s_per_step": cpu_sample(args.workload, args.cpu_steps)}))
@@ -76,7 +80,8 @@ def main():
         return bench_remd(args)
     s, p = W.config(args.workload)
     dt_ps = 0.001
-    e = Engine(s, p)
+    shape, trials = autotune(args, s, p, dt_ps)
+    e = Engine(s, p, shape=shape)
     e.set_positions(s.r)
     e.set_velocities(s.v)
     e.md_setup(dt=dt_ps, bond=1.0, angle=109.47, tol=1e-8)
@@ -127,7 +132,9 @@ def main():
                    "box_A": s.boxl, "rc_A": p["rc"], "skin_A": round(p["skin"], 5),
                    "kspace": "P3M" if p["kmode"] == 1 else "Ewald",
                    "mesh": list(p.get("mesh", [])), "order": p.get("order"),
-                   "eta": p["eta"], "dt_fs": dt_ps * 1e3, "graph": not args.no_graph},
+                   "eta": p["eta"], "dt_fs": dt_ps * 1e3, "graph": not args.no_graph,
+                   "shape": shape},
+        "autotune": trials,
         "pair_interactions_per_s": {"per_pair_kernel": p_c / t_pair,
                                     "per_full_step": p_c / (ms_step * 1e-3),
                                     "site_pairs": p_c, "lj_pairs": p_lj, "group_pairs": g_rc},
@@ -235,10 +242,11 @@ def bench_remd(args):
         return {k: v for k, v in c.items() if k in keep}
 
     systems, params, kT, ladders = W.remd_batch(nrep=16, kspace=ks)
-    e = Engine(systems, params)
+    dt_ps = 0.001
+    shape, trials = autotune(args, systems, params, dt_ps)
+    e = Engine(systems, params, shape=shape)
     e.set_positions(np.concatenate([s.r for s in systems]))
     e.set_velocities(np.concatenate([s.v for s in systems]))
-    dt_ps = 0.001
     e.md_setup(dt=dt_ps)
     e.remd_setup(ladders, kT, seed=11)
     interval = 100
@@ -267,12 +275,24 @@ def bench_remd(args):
                                for k, v in params[i].items()
                                if k in ("kmode", "eta", "kc", "mesh", "order")}
                               for i in (0, 16, 32)],
-                   "exchange_interval": interval, "dt_fs": dt_ps * 1e3},
+                   "exchange_interval": interval, "dt_fs": dt_ps * 1e3, "shape": shape},
+        "autotune": trials,
         "atom_steps_per_s": natoms * 1e3 / ms_step,
         "exchange_acceptance": [round(float(st[a, 1]) / max(1, st[a, 0]), 3)
                                 for lad in ladders for a in lad[:-1]],
     }
     print(json.dumps(out))
+
+
+def autotune(args, systems, params, dt_ps):
+    """The engine shape for the timed run: the autotuner's choice from timed
+    MD blocks of this workload (before the timed region), or the default
+    with --no-tune.  Returns (shape, [{shape, ms_per_step} per trial])."""
+    from molmodel_amd import engine as E
+    if args.no_tune or args.no_graph:
+        return E.default_shape(), []
+    best, trials = E.tune_shape(systems, params, dt=dt_ps, steps=args.tune_steps)
+    return best, [dict(shape=t, ms_per_step=round(ms, 5)) for t, ms in trials]
 
 
 def _cpu_name():

@@ -213,6 +213,48 @@ const char *mm_last_error(void);
 int mm_device_count(void);
 
 int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out);
+
+/* How a batch is laid onto the device (no reference counterpart: the
+ * reference has one CPU loop).  None of these changes what is computed:
+ *   cell_groups   mean groups per neighbour-list cell; 2 is the reference's
+ *                 cell count (nbrlist.cpp:11), for which the pair set equals
+ *                 the reference's bit for bit; other sizes give the same set
+ *                 except pairs whose FP64 distance rounds across rc + skin
+ *                 (outside the cutoff: no force changes), and forces to
+ *                 FP64 summation order;
+ *   pair_slots    pair-kernel workgroups per CU (1..4; with fewer than 4,
+ *                 the rest of each CU stays free for the reciprocal-space
+ *                 kernels beside it); forces bit-identical;
+ *   spread_block  P3M spreading block side in mesh columns (the smallest
+ *                 factor of the mesh side >= it); mesh bit-identical;
+ *   graph_steps   MD steps captured per graph (1..64); trajectories
+ *                 bit-identical.
+ * mm_shape_default: 2, 3, 2, 10 (MOLMODEL_AMD_PAIR_SLOTS,
+ * MOLMODEL_AMD_SPREAD_B and MOLMODEL_AMD_GRAPH_STEPS override the last
+ * three); mm_engine_create uses it. */
+typedef struct {
+  int cell_groups, pair_slots, spread_block, graph_steps;
+} mm_shape;
+void mm_shape_default(mm_shape *s);
+int mm_engine_create_shaped(int nsys, const mm_system_desc *sys, const mm_shape *shape,
+                            mm_engine **out);
+
+/* The autotuner: builds a fresh engine per candidate shape from the
+ * caller's batch (positions r, velocities v or NULL), runs MD with
+ * mm_md_setup(dt, bond, angle_deg, tol), and times `steps`-step blocks
+ * (graph replay; the best of two after a warm-up); one choice at a time
+ * from the default shape: pair_slots {2, 3, 4}, spread_block {2, 4} (when a
+ * system uses P3M), cell_groups {1, 2, 4}, graph_steps {5, 10, 20}.  *best:
+ * the fastest; trials[0..max_trials) the shapes timed and their ms per
+ * step (-1 for a shape that could not run), *ntrials their number.  The
+ * caller's own engines are not touched. */
+typedef struct {
+  mm_shape shape;
+  double ms_per_step;
+} mm_shape_trial;
+int mm_tune_shape(int nsys, const mm_system_desc *sys, const double *r, const double *v,
+                  double dt, double bond, double angle_deg, double tol, int steps,
+                  mm_shape *best, mm_shape_trial *trials, int max_trials, int *ntrials);
 int mm_engine_destroy(mm_engine *e);
 /* totals over all systems of the batch */
 int mm_engine_sizes(const mm_engine *e, int *nsite, int *ngroup);

@@ -23,7 +23,7 @@ ARCH = os.environ.get("MM_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["engine.cpp", "capi.cpp", "refapi.cpp", "nbrlist.hip", "rspace.hip", "kspace.hip",
            "pppm.hip", "finish.hip", "remd.hip", "tune.cpp", "adjoint.cpp", "adjoint.hip",
-           "noalloc.hip", "sites.cpp", "msimio.cpp"]
+           "noalloc.hip", "sites.cpp", "msimio.cpp", "shape.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-x", "hip",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", "-I", os.path.join(HERE, "..", "include")]

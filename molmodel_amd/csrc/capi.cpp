@@ -77,12 +77,45 @@ static int desc_inputs(int nsys, const mm_system_desc *sys, std::vector<SystemIn
   return MM_OK;
 }
 
+// The default shape: two groups per cell (the reference's cell count),
+// three pair workgroups per CU, 2 x 2 spreading column blocks, ten MD steps
+// per graph; MOLMODEL_AMD_PAIR_SLOTS, MOLMODEL_AMD_SPREAD_B and
+// MOLMODEL_AMD_GRAPH_STEPS override the last three.
+void mm_shape_default(mm_shape *s) {
+  if (!s) return;
+  s->cell_groups = 2;
+  s->pair_slots = 3;
+  s->spread_block = 2;
+  s->graph_steps = 10;
+  if (const char *v = getenv("MOLMODEL_AMD_PAIR_SLOTS")) s->pair_slots = atoi(v);
+  if (const char *v = getenv("MOLMODEL_AMD_SPREAD_B")) s->spread_block = atoi(v);
+  if (const char *v = getenv("MOLMODEL_AMD_GRAPH_STEPS")) s->graph_steps = atoi(v);
+}
+
+static int check_shape(const mm_shape &s) {
+  if (s.cell_groups < 1 || s.cell_groups > 8 || s.pair_slots < 1 || s.pair_slots > 4 ||
+      s.spread_block < 1 || s.spread_block > 8 || s.graph_steps < 1 || s.graph_steps > 64) {
+    set_last_error("mm_shape: cell_groups 1..8, pair_slots 1..4, spread_block 1..8, "
+                   "graph_steps 1..64");
+    return MM_ERR_ARG;
+  }
+  return MM_OK;
+}
+
 int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out) {
-  if (!out || nsys <= 0 || !sys) {
+  mm_shape s;
+  mm_shape_default(&s);
+  return mm_engine_create_shaped(nsys, sys, &s, out);
+}
+
+int mm_engine_create_shaped(int nsys, const mm_system_desc *sys, const mm_shape *shape,
+                            mm_engine **out) {
+  if (!out || nsys <= 0 || !sys || !shape) {
     set_last_error("mm_engine_create: bad arguments");
     return MM_ERR_ARG;
   }
   *out = nullptr;
+  TRY(check_shape(*shape));
   if (mm_device_count() <= 0) {
     set_last_error("mm_engine_create: no HIP device");
     return MM_ERR_HIP;
@@ -90,6 +123,10 @@ int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out) {
   std::vector<SystemInput> in;
   TRY(desc_inputs(nsys, sys, in));
   mm_engine *m = new mm_engine();
+  m->e.cell_groups = shape->cell_groups;
+  m->e.pair_slots = shape->pair_slots;
+  m->e.spread_block = shape->spread_block;
+  m->e.graph_steps = shape->graph_steps;
   int rc = engine_create(m->e, in);
   if (rc) {
     engine_destroy(m->e);
@@ -108,6 +145,11 @@ int mm_check_layout(int nsys, const mm_system_desc *sys) {
   std::vector<SystemInput> in;
   TRY(desc_inputs(nsys, sys, in));
   Engine e;
+  mm_shape s;
+  mm_shape_default(&s);
+  TRY(check_shape(s));
+  e.cell_groups = s.cell_groups;
+  e.spread_block = s.spread_block;
   HostLayout h;
   if (engine_layout(e, in, h)) return MM_ERR_ARG;
   if (engine_check_layout(e, h)) return MM_ERR_ARG;
@@ -695,13 +737,6 @@ static int enqueue_step(mm_engine *m, int mode) {
   return 0;
 }
 
-// steps per captured MD graph (MOLMODEL_AMD_GRAPH_STEPS, default 10: 549
-// vs 538 ns/day for one step per graph on the 98k-atom P3M box)
-static int graph_block_steps() {
-  const char *v = getenv("MOLMODEL_AMD_GRAPH_STEPS");
-  const int k = v ? atoi(v) : 10;
-  return k < 1 ? 1 : (k > 64 ? 64 : k);
-}
 
 // nsteps of velocity Verlet, queued without waiting for the device
 static int md_run_async(mm_engine *m, int nsteps, int use_graph) {
@@ -715,7 +750,9 @@ static int md_run_async(mm_engine *m, int nsteps, int use_graph) {
   if (use_graph && full > 0) {
     // the step graph: neighbour-list check/rebuild, forces, finish; blocks
     // of K steps captured as one graph save the launch gap between graphs
-    const int K = graph_block_steps();
+    // steps per captured graph (the shape's graph_steps; ten gave 549 vs
+    // 538 ns/day for one on the 98k-atom P3M box)
+    const int K = e.graph_steps;
     auto capture = [&](int steps, hipGraph_t *graph, hipGraphExec_t *exec) -> int {
       MM_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
       int rc = 0;

@@ -26,7 +26,11 @@ const char *last_error() { return g_last_error.c_str(); }
 static double sq(double x) { return x * x; }
 
 // ---- neighbor-list geometry (nbrlist.cpp:11, 96-102, 105-120) ----
-int nl_ncell(int ngroup) { return (int)floor(cbrt((double)(ngroup / 2 + 3))); }
+// cells per side for a mean of cell_groups groups per cell; 2 is the
+// reference's count (nbrlist.cpp:11)
+int nl_ncell(int ngroup, int cell_groups) {
+  return (int)floor(cbrt((double)(ngroup / cell_groups + 3)));
+}
 
 int nl_nmax(int ncell, double boxl, double rc, double skin) {
   const double lc = boxl * (1.0 / ncell);
@@ -456,7 +460,7 @@ int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h) 
     // neutralising background (eleclj.cpp:970-975)
     P.bg_phi = background_phi(S, S.eta, P.volume);
     // neighbor list
-    P.ncell = nl_ncell(S.ngroup);
+    P.ncell = nl_ncell(S.ngroup, e.cell_groups);
     P.lc = S.boxl * (1.0 / P.ncell);
     P.cell0 = cell0;
     cell0 += P.ncell * P.ncell * P.ncell;
@@ -547,9 +551,7 @@ int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h) 
           const double a = -2 * M_PI * (t % nn[d]) / nn[d];
           twid.push_back(make_float2((float)cos(a), (float)sin(a)));
         }
-      int want = 2;
-      if (const char *v = getenv("MOLMODEL_AMD_SPREAD_B")) want = std::max(2, atoi(v));
-      MM_REQ(choose_spread_blocks(S.nx, S.ny, S.nz, 4096, 4096, P.bx, P.by, want),
+      MM_REQ(choose_spread_blocks(S.nx, S.ny, S.nz, 4096, 4096, P.bx, P.by, e.spread_block),
              "mesh dimensions give no valid spreading column blocks");
       P.nbx = S.nx / P.bx;
       P.nby = S.ny / P.by;
@@ -745,18 +747,16 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   MM_TRY(dalloc(&e.d_fpart, (size_t)e.nsys * e.nblk_part * 16));
   MM_TRY(dalloc(&e.d_done, 4 * (size_t)e.nsys));
   {
-    // persistent pair grid: MOLMODEL_AMD_PAIR_SLOTS workgroups per CU (the
-    // kernel's occupancy allows 4; the default 3 leaves one slot on every
-    // CU to the reciprocal-space kernels of the other stream)
+    // persistent pair grid: pair_slots workgroups per CU (the kernel's
+    // occupancy allows 4; the default 3 leaves one slot on every CU to the
+    // reciprocal-space kernels of the other stream)
     std::vector<int> ts(e.nsys + 1, 0);
     for (int s = 0; s < e.nsys; s++) ts[s + 1] = ts[s] + pair_tiles(e.hsp[s].ngroup);
     e.ntile = ts[e.nsys];
     int dev = 0, ncu = 256;
     MM_HIP(hipGetDevice(&dev));
     MM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    int slots = 3;
-    if (const char *v = getenv("MOLMODEL_AMD_PAIR_SLOTS")) slots = std::max(1, std::min(8, atoi(v)));
-    const int want = std::min(slots * ncu, (e.ntile + 7) / 8 * 8);
+    const int want = std::min(e.pair_slots * ncu, (e.ntile + 7) / 8 * 8);
     e.pair_blocks = std::max(8, want / 8 * 8);
     MM_TRY(dalloc(&e.d_tile_start, e.nsys + 1));
     MM_TRY(dput(e.d_tile_start, ts));

@@ -130,6 +130,10 @@ struct Engine {
   int any_water = 0;          // some system holds only three-site waters (SysParams::water)
   int ntile = 0;
   int pair_blocks = 0;        // persistent grid (a multiple of 8)
+  // shape (mm_shape): mean groups per neighbour-list cell (2: the
+  // reference's cell count), pair workgroups per CU, spreading column-block
+  // side in mesh columns, MD steps per captured graph
+  int cell_groups = 2, pair_slots = 3, spread_block = 2, graph_steps = 10;
   int *d_tile_start = nullptr;   // nsys + 1
   int nblk_finish = 0;         // finish blocks per system (max)
   int nblk_part = 0;           // partial-sum slots per system

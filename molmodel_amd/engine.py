@@ -80,6 +80,10 @@ def lib():
         L.mm_desc_init.argtypes = [_P]
         L.mm_desc_resolve.argtypes = [_P, _D, _D]
         L.mm_derive_sites.argtypes = [_I, _P, _P, _P, _P, _P, _P, _L, _P, _P]
+        L.mm_shape_default.restype = None
+        L.mm_shape_default.argtypes = [_P]
+        L.mm_engine_create_shaped.argtypes = [_I, _P, _P, C.POINTER(_P)]
+        L.mm_tune_shape.argtypes = [_I, _P, _P, _P, _D, _D, _D, _D, _I, _P, _P, _I, _P]
         L.mm_free.restype = None
         L.mm_free.argtypes = [_P]
         L.mm_msim_preprocess.argtypes = [_I, _P, C.POINTER(_P)]
@@ -98,6 +102,65 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(_P) if a is not None else None
+
+
+class Shape(C.Structure):
+    _fields_ = [("cell_groups", _I), ("pair_slots", _I), ("spread_block", _I),
+                ("graph_steps", _I)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ShapeTrial(C.Structure):
+    _fields_ = [("shape", Shape), ("ms_per_step", _D)]
+
+
+def default_shape():
+    s = Shape()
+    lib().mm_shape_default(C.byref(s))
+    return s.as_dict()
+
+
+def _shape(d):
+    s = Shape()
+    lib().mm_shape_default(C.byref(s))
+    for k, v in (d or {}).items():
+        setattr(s, k, int(v))
+    return s
+
+
+def _descs(systems, params):
+    if not isinstance(systems, (list, tuple)):
+        systems, params = [systems], [params]
+    elif not isinstance(params, (list, tuple)):
+        params = [params] * len(systems)
+    descs, keep = [], []
+    for s, p in zip(systems, params):
+        d, k = system_desc(s, p)
+        descs.append(d)
+        keep.append(k)
+    return list(systems), (SystemDesc * len(descs))(*descs), keep
+
+
+def tune_shape(systems, params, dt=0.001, bond=1.0, angle=109.47, tol=1e-8, steps=100,
+               r=None, v=None):
+    """The autotuner (mm_tune_shape): the fastest shape for this batch from
+    timed MD blocks on the device; returns (best shape dict, trials list of
+    (shape dict, ms per step))."""
+    systems, arr, keep = _descs(systems, params)
+    r = np.ascontiguousarray(np.concatenate([s.r for s in systems]) if r is None else r,
+                             dtype=np.float64).reshape(-1)
+    v = np.ascontiguousarray(np.concatenate([s.v for s in systems]) if v is None else v,
+                             dtype=np.float64).reshape(-1)
+    best = Shape()
+    trials = (ShapeTrial * 32)()
+    nt = C.c_int(0)
+    _check(lib().mm_tune_shape(len(systems), C.cast(arr, _P), _ptr(r), _ptr(v), dt, bond, angle,
+                               tol, int(steps), C.byref(best), C.cast(trials, _P), 32,
+                               C.byref(nt)))
+    return best.as_dict(), [(trials[i].shape.as_dict(), trials[i].ms_per_step)
+                            for i in range(min(nt.value, 32))]
 
 
 class MMError(RuntimeError):
@@ -286,22 +349,19 @@ def system_desc(system, params):
 class Engine:
     """A batch of systems resident on the GPU."""
 
-    def __init__(self, systems, params):
-        if not isinstance(systems, (list, tuple)):
-            systems, params = [systems], [params]
-        elif not isinstance(params, (list, tuple)):
-            params = [params] * len(systems)
-        self.systems = list(systems)
-        descs, self._keep = [], []
-        for s, p in zip(systems, params):
-            d, k = system_desc(s, p)
-            descs.append(d)
-            self._keep.append(k)
-        arr = (SystemDesc * len(descs))(*descs)
+    def __init__(self, systems, params, shape=None):
+        """shape: a dict of mm_shape fields (the rest default), or None"""
+        systems, arr, self._keep = _descs(systems, params)
+        self.systems = systems
         h = _P()
-        _check(lib().mm_engine_create(len(descs), C.cast(arr, _P), C.byref(h)))
+        if shape is None:
+            _check(lib().mm_engine_create(len(systems), C.cast(arr, _P), C.byref(h)))
+        else:
+            sh = _shape(shape)
+            _check(lib().mm_engine_create_shaped(len(systems), C.cast(arr, _P), C.byref(sh),
+                                                 C.byref(h)))
         self.h = h
-        self.nsys = len(descs)
+        self.nsys = len(systems)
         self.nsite = sum(s.nsite for s in systems)
         self.offsets = np.cumsum([0] + [s.nsite for s in systems])
 

@@ -64,3 +64,16 @@ def test_invalid_input_is_reported():
     s, p = W.config("water648_ewald")
     rc, msg = check([s], [dict(p, rc=0.6 * s.boxl)])
     assert rc < 0 and "0.49" in msg
+
+
+def test_default_shape_and_shape_checks():
+    assert E.default_shape() == dict(cell_groups=2, pair_slots=3, spread_block=2, graph_steps=10)
+    s, p = W.config("water648_ewald")
+    d, keep = E.system_desc(s, p)
+    h = C.c_void_p()
+    L = E.lib()
+    for bad in (dict(pair_slots=0), dict(pair_slots=5), dict(cell_groups=0),
+                dict(spread_block=9), dict(graph_steps=65)):
+        sh = E._shape(bad)
+        rc = L.mm_engine_create_shaped(1, C.byref(d), C.byref(sh), C.byref(h))
+        assert rc == -1 and "mm_shape" in L.mm_last_error().decode()
