@@ -317,6 +317,11 @@ extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, in
   const SystemInput &S = e.in[sys];
   const SysParams &P = e.hsp[sys];
   const int N = S.nsite, NT = e.nsite;
+  if (P.kmode == 1 && P.ik) {
+    set_last_error("mm_param_gradients: the P3M adjoint is built for real-space differentiation; "
+                   "use mm_param_gradients_basis for a system with ik");
+    return MM_ERR_ARG;
+  }
   // site types: distinct (q, sigma, epsilon, mask), in order of appearance
   std::map<std::tuple<double, double, double, int>, int> tmap;
   std::vector<int> ty(N);

@@ -221,7 +221,7 @@ int mm_desc_resolve(mm_system_desc *d, double p3m_grid_spacing, double p3m_grid_
     if (d->eta < 0) {
       const double latv[9] = {L, 0, 0, 0, L, 0, 0, 0, L};
       double cr = 0, ck = 0;
-      p3m_optimize_eta(latv, d->mesh, d->nalias, d->order, h, d->rc, &d->eta, &cr, &ck);
+      p3m_optimize_eta(latv, d->mesh, d->nalias, d->order, d->ik, h, d->rc, &d->eta, &cr, &ck);
     }
   } else if (d->eta < 0) {
     d->eta = sqrt(M_PI * d->kc / (d->rc * L));   // eleclj.cpp:229-230

@@ -210,9 +210,12 @@ static double sincpi(double x) {   // pppm.c:56-59
   return fabs(y) < 1e-16 ? 1 - sq(y) / 6 : sin(y) / y;
 }
 
-// optimal influence function at signed index (pppm.c:331-388)
+// optimal influence function at signed index (pppm.c:331-388): for the
+// field by -i k on the mesh (ik) the numerator sums g u^2 (k . k_p) and the
+// denominator is (sum u^2)^2 k^2; for real-space differentiation they are
+// sum g u^2 k_p^2 and (sum u^2)(sum u^2 k_p^2)
 double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl,
-                double inv_4eta2, double inv_v, int nalias, int order, double *chi_acc) {
+                double inv_4eta2, double inv_v, int nalias, int order, int ik, double *chi_acc) {
   if (jx == 0 && jy == 0 && jz == 0) return 0;
   double ux[16], uy[16], uz[16];
   for (int qq = -nalias; qq <= nalias; qq++) {
@@ -220,6 +223,9 @@ double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl
     uy[qq + nalias] = pow(sincpi((double)jy / ny + qq), 2 * order);
     uz[qq + nalias] = pow(sincpi((double)jz / nz + qq), 2 * order);
   }
+  const double kx = rl[0] * jx + rl[3] * jy + rl[6] * jz;
+  const double ky = rl[1] * jx + rl[4] * jy + rl[7] * jz;
+  const double kz = rl[2] * jx + rl[5] * jy + rl[8] * jz;
   double numer = 0, d1 = 0, d2 = 0, chi = 0;
   for (int px = -nalias; px <= nalias; px++) {
     const double pxx = jx + px * nx;
@@ -234,15 +240,20 @@ double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl
         const double kp2 = k0 * k0 + k1 * k1 + k2 * k2;
         const double u2 = ux[px + nalias] * uy[py + nalias] * uz[pz + nalias];
         const double gg = 4 * M_PI / kp2 * exp(-kp2 * inv_4eta2);
-        numer += gg * kp2 * u2;
+        if (ik) {
+          numer += gg * u2 * (kx * k0 + ky * k1 + kz * k2);
+        } else {
+          numer += gg * kp2 * u2;
+          d2 += u2 * kp2;
+        }
         d1 += u2;
-        d2 += u2 * kp2;
         chi += gg * gg * kp2;
       }
     }
   }
-  if (chi_acc) *chi_acc += chi - sq(numer) / (d1 * d2);
-  return inv_v * numer / (d1 * d2);
+  const double denom = ik ? sq(d1) * (kx * kx + ky * ky + kz * kz) : d1 * d2;
+  if (chi_acc) *chi_acc += chi - sq(numer) / denom;
+  return inv_v * numer / denom;
 }
 
 static bool radix_ok(int n) {
@@ -515,20 +526,22 @@ int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h) 
       P.nk = nk;
       k0 += nk;
     } else if (S.kmode == 1) {
-      MM_REQ(S.ik == 0, "P3M ik differentiation is not supported on the device (use real-space differentiation)");
+      MM_REQ(S.ik == 0 || S.ik == 1, "ik must be 0 (real-space differentiation) or 1 (-i k on the mesh)");
       MM_REQ(S.order >= 1 && S.order <= 7, "assignment order must be 1..7");
       MM_REQ(radix_ok(S.nx) && radix_ok(S.ny) && radix_ok(S.nz),
              "mesh dimensions must be <= 128 with prime factors 2,3,5,7");
       MM_REQ(S.nx >= 8 && S.ny >= 8 && S.nz >= 8, "mesh dimensions must be >= 8");
       MM_REQ(S.nalias >= 0 && S.nalias <= 7, "alias sum limit must be 0..7");
       P.nx = S.nx; P.ny = S.ny; P.nz = S.nz; P.order = S.order;
+      P.ik = S.ik;
       for (int a = 0; a < 9; a++) P.T[a] = inv[a];
       P.T[0] *= S.nx; P.T[3] *= S.nx; P.T[6] *= S.nx;
       P.T[1] *= S.ny; P.T[4] *= S.ny; P.T[7] *= S.ny;
       P.T[2] *= S.nz; P.T[5] *= S.nz; P.T[8] *= S.nz;
       const int nzh = S.nz / 2 + 1;
-      P.mesh0 = mesh0; mesh0 += (long)S.nx * S.ny * S.nz;
-      P.cmesh0 = cmesh0; cmesh0 += (long)S.nx * S.ny * nzh;
+      const int ncomp = S.ik ? 4 : 1;   // phi [, Ex, Ey, Ez]
+      P.mesh0 = mesh0; mesh0 += ncomp * (long)S.nx * S.ny * S.nz;
+      P.cmesh0 = cmesh0; cmesh0 += ncomp * (long)S.nx * S.ny * nzh;
       P.green0 = green0;
       green0 += (long)S.nx * S.ny * nzh;
       const double inv_4eta2 = 1 / sq(2 * S.eta), inv_v = 1.0 / P.volume;
@@ -540,7 +553,7 @@ int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h) 
           for (int iz = 0; iz < nzh; iz++)
             gh[((size_t)iy * nzh + iz) * S.nx + ix] =
                 (float)green_at(jx, jy, iz, S.nx, S.ny, S.nz, P.RL, inv_4eta2, inv_v, S.nalias, S.order,
-                                nullptr);
+                                S.ik, nullptr);
         }
       }
       green.insert(green.end(), gh.begin(), gh.end());
@@ -622,9 +635,9 @@ int engine_check_layout(const Engine &e, const HostLayout &h) {
     lj.push_back({P.lj0, (long)P.ntype * P.ntype, s});
     if (P.kmode == 0) kv.push_back({4L * P.k0, 4L * P.nk, s});
     if (P.kmode == 1) {
-      const long nzh = P.nz / 2 + 1;
-      mesh.push_back({P.mesh0, (long)P.nx * P.ny * P.nz, s});
-      cmesh.push_back({P.cmesh0, (long)P.nx * P.ny * nzh, s});
+      const long nzh = P.nz / 2 + 1, ncomp = P.ik ? 4 : 1;
+      mesh.push_back({P.mesh0, ncomp * P.nx * P.ny * P.nz, s});
+      cmesh.push_back({P.cmesh0, ncomp * P.nx * P.ny * nzh, s});
       gr.push_back({P.green0, (long)P.nx * P.ny * nzh, s});
       tw.push_back({P.twid0, 3L * 128, s});
       bin.push_back({P.bin0, (long)P.nbx * P.nby, s});

@@ -201,11 +201,11 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in);
 // host helpers shared with the reference-named entry points
 void tensor_inverse(double *a, const double *b);
 double green_at(int jx, int jy, int jz, int nx, int ny, int nz, const double *rl,
-                double inv_4eta2, double inv_v, int nalias, int order, double *chi_acc);
+                double inv_4eta2, double inv_v, int nalias, int order, int ik, double *chi_acc);
 int multiple_of_2357_near(double x);
-double p3m_influence(const double latv[9], const int ngrid[3], int nalias, int order, double eta,
-                     std::vector<double> *green);
-void p3m_optimize_eta(const double latv[9], const int ngrid[3], int nalias, int order,
+double p3m_influence(const double latv[9], const int ngrid[3], int nalias, int order, int ik,
+                     double eta, std::vector<double> *green);
+void p3m_optimize_eta(const double latv[9], const int ngrid[3], int nalias, int order, int ik,
                       double grid_spacing, double rc, double *eta, double *chi_rspace,
                       double *chi_p3m);
 void cheb_fit(double a, double b, double (*f)(double), double *c4);

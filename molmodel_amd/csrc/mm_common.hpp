@@ -62,6 +62,9 @@ struct SysParams {
   int bx, by, nbx, nby, bin0;  // spread column blocks (bx x by mesh columns each)
   int twid0;                   // first of this system's 3 x 128 FFT twiddles
   int water;                   // every group is a three-site water (mask bytes WATER_MASKS)
+  int ik;                      // P3M field by -i k on the mesh: four real meshes (phi, Ex, Ey,
+                               // Ez) at mesh0 + c nx ny nz and four half spectra at
+                               // cmesh0 + c nx ny (nz/2+1), c = 0..3
   // dynamics
   double dt;
   double bg_phi;               // neutralising background potential (or 0)

@@ -310,8 +310,14 @@ __device__ __forceinline__ void fft_stage(const float2 *xl, float2 *yl, int n, i
 // Stockham autosort FFT of FFT_LINES lines of length n held in x (line l at
 // x + l*FFT_MAXN); every thread of the block must call it.  Radices are
 // taken greedily (4, 2, 3, 5, 7).  Returns the buffer holding the result.
+// Always inlined: as a called function (the compiler's choice for the
+// mixed-radix build) it took a call and 192-416 bytes of scratch per lane in
+// every kernel that uses it, and the x pass with virial and the field on
+// the mesh gave wrong results in that form; inlined, the mixed-radix
+// kernels fit in registers (at most 32 bytes of spill).
 template <bool POW2>
-__device__ float2 *lds_fft_lines(float2 *x, float2 *y, int n, const float2 *tw, int sign) {
+__device__ __forceinline__ float2 *lds_fft_lines(float2 *x, float2 *y, int n, const float2 *tw,
+                                                 int sign) {
   const int l = threadIdx.x / FFT_T, tid = threadIdx.x % FFT_T;
   int p = 1, rem = n;
   while (rem > 1) {
@@ -336,7 +342,9 @@ __device__ __forceinline__ void load_tw(float2 *stw, const float2 *tw, int n) {
 }
 
 // pass 0: z lines, real -> half spectrum.  pass 1/3: y lines fwd/inv.
-// pass 4: z lines, half spectrum -> real.
+// pass 4: z lines, half spectrum -> real.  blockIdx.z: the mesh component
+// (phi, then Ex, Ey, Ez of a system whose field is formed as -i k phi on
+// the mesh; the forward passes run over component 0 only).
 template <bool POW2>
 __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restrict__ sp,
                                                      const float2 *__restrict__ twid_all,
@@ -347,7 +355,11 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
   const int s = blockIdx.y;
   const SysParams &P = sp[s];
   if (P.kmode != 1) return;
+  const int comp = blockIdx.z;
+  if (comp > 0 && !P.ik) return;
   const int nx = P.nx, ny = P.ny, nz = P.nz, nzh = nz / 2 + 1;
+  const long mesh0 = P.mesh0 + comp * (long)nx * ny * nz;
+  const long cmesh0 = P.cmesh0 + comp * (long)nx * ny * nzh;
   int nlines, n, twoff;
   if (pass == 0 || pass == 4) { nlines = nx * ny; n = nz; twoff = 2 * FFT_MAXN; }
   else { nlines = nx * nzh; n = ny; twoff = FFT_MAXN; }
@@ -361,12 +373,12 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
   // load
   if (pass == 0) {
     if (on) {
-      const float *src = mesh + P.mesh0 + (long)line * nz;
+      const float *src = mesh + mesh0 + (long)line * nz;
       for (int t = tid; t < n; t += FFT_T) xl[t] = make_float2(src[t], 0.f);
     }
   } else if (pass == 4) {
     if (on) {
-      const float2 *src = cmesh + P.cmesh0 + (long)line * nzh;
+      const float2 *src = cmesh + cmesh0 + (long)line * nzh;
       for (int t = tid; t < nzh; t += FFT_T) xl[t] = src[t];
     }
     __syncthreads();
@@ -378,7 +390,7 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
   } else {
     if (on) {
       const int xi = line / nzh, kz = line % nzh;
-      const float2 *src = cmesh + P.cmesh0 + (long)xi * ny * nzh + kz;
+      const float2 *src = cmesh + cmesh0 + (long)xi * ny * nzh + kz;
       for (int t = tid; t < n; t += FFT_T) xl[t] = src[(long)t * nzh];
     }
   }
@@ -387,14 +399,14 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
   float2 *rl = res + l * FFT_MAXN;
   if (!on) return;
   if (pass == 0) {
-    float2 *dst = cmesh + P.cmesh0 + (long)line * nzh;
+    float2 *dst = cmesh + cmesh0 + (long)line * nzh;
     for (int t = tid; t < nzh; t += FFT_T) dst[t] = rl[t];
   } else if (pass == 4) {
-    float *dst = mesh + P.mesh0 + (long)line * nz;
+    float *dst = mesh + mesh0 + (long)line * nz;
     for (int t = tid; t < n; t += FFT_T) dst[t] = rl[t].x;
   } else {
     const int xi = line / nzh, kz = line % nzh;
-    float2 *dst = cmesh + P.cmesh0 + (long)xi * ny * nzh + kz;
+    float2 *dst = cmesh + cmesh0 + (long)xi * ny * nzh + kz;
     for (int t = tid; t < n; t += FFT_T) dst[(long)t * nzh] = rl[t];
   }
 }
@@ -403,7 +415,10 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
 // Also the mesh energy 1/2 sum G |rho_k|^2 and its virial
 // W_ab = sum e_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 eta^2))); ENER false:
 // the convolution alone (inner MD steps, whose energies are not formed).
-template <bool VIR, bool POW2, bool ENER = true>
+// IK: systems whose field is formed on the mesh also get E_a = -i k_a phi_k
+// (k from the signed x, y index and the half-spectrum z index,
+// pppm.c:577-594), each inverse-transformed along x into its own component.
+template <bool VIR, bool POW2, bool ENER = true, bool IK = false>
 __global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__restrict__ sp,
                                                        const float2 *__restrict__ twid_all,
                                                        const float *__restrict__ green,
@@ -463,11 +478,48 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__res
       rl[t] = make_float2(v.x * gg, v.y * gg);
     }
   }
-  __syncthreads();
-  res = lds_fft_lines<POW2>(res, oth, nx, stw, +1);
-  if (on) {
+  if constexpr (IK) {
+    // each thread keeps its own points of phi_k in registers and rebuilds
+    // the line in LDS for every component
+    constexpr int NK = FFT_MAXN / FFT_T;
+    float2 keep[NK];
     float2 *rl = res + l * FFT_MAXN;
-    for (int t = tid; t < nx; t += FFT_T) src[t * xstride] = rl[t];
+#pragma unroll
+    for (int j = 0; j < NK; j++) {
+      const int t = tid + j * FFT_T;
+      keep[j] = (on && t < nx) ? rl[t] : make_float2(0.f, 0.f);
+    }
+    const int jy = ky > ny / 2 ? ky - ny : ky;
+    const int ncomp = P.ik ? 4 : 1;
+    const long cstride = (long)nx * ny * nzh;
+    for (int c = 0; c < ncomp; c++) {
+      if (c > 0) {
+        const int a = c - 1;
+#pragma unroll
+        for (int j = 0; j < NK; j++) {
+          const int t = tid + j * FFT_T;
+          if (on && t < nx) {
+            const int jx = t > nx / 2 ? t - nx : t;
+            const float k = (float)(P.RL[a] * jx + P.RL[3 + a] * jy + P.RL[6 + a] * kz);
+            rl[t] = make_float2(k * keep[j].y, -k * keep[j].x);
+          }
+        }
+      }
+      __syncthreads();
+      float2 *out = lds_fft_lines<POW2>(res, oth, nx, stw, +1);
+      if (on) {
+        const float2 *ol = out + l * FFT_MAXN;
+        for (int t = tid; t < nx; t += FFT_T) src[c * cstride + t * xstride] = ol[t];
+      }
+      __syncthreads();   // the next component rebuilds its line in these buffers
+    }
+  } else {
+    __syncthreads();
+    res = lds_fft_lines<POW2>(res, oth, nx, stw, +1);
+    if (on) {
+      float2 *rl = res + l * FFT_MAXN;
+      for (int t = tid; t < nx; t += FFT_T) src[t * xstride] = rl[t];
+    }
   }
   if constexpr (!ENER) return;
   // deterministic block partials, reduced by the last block
@@ -482,7 +534,10 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_xgreen(const SysParams *__res
   }
 }
 
-template <int P>
+// IK: systems whose field was formed on the mesh; phi and E_a are
+// interpolated from their four meshes with the assignment weights
+// (pppm.c:618-626), E already Cartesian
+template <int P, bool IK = false>
 __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict__ sp,
                                                     const int4 *__restrict__ sgrid,
                                                     const float4 *__restrict__ sfrac,
@@ -491,7 +546,7 @@ __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict_
                                                     int *__restrict__ fill) {
   const int s = blockIdx.y;
   const SysParams &Q = sp[s];
-  if (Q.kmode != 1 || Q.order != P) return;
+  if (Q.kmode != 1 || Q.order != P || Q.ik != (int)IK) return;
   if (blockIdx.x == 0)   // the spreading has read the bin counters: clear them
     for (int t = threadIdx.x; t < Q.nbx * Q.nby; t += blockDim.x) fill[Q.bin0 + t] = 0;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -502,9 +557,9 @@ __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict_
   const int nx = Q.nx, ny = Q.ny, nz = Q.nz;
   constexpr int istart = -((P + 1) / 2 - 1);
   float wx[P], wy[P], wz[P], dx[P], dy[P], dz[P];
-  bspline_w<P>(f.x, wx, dx);
-  bspline_w<P>(f.y, wy, dy);
-  bspline_w<P>(f.z, wz, dz);
+  bspline_w<P>(f.x, wx, IK ? nullptr : dx);
+  bspline_w<P>(f.y, wy, IK ? nullptr : dy);
+  bspline_w<P>(f.z, wz, IK ? nullptr : dz);
   int ix[P], iy[P], iz[P];
 #pragma unroll
   for (int a = 0; a < P; a++) {
@@ -514,6 +569,30 @@ __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict_
   }
   const float *m0 = mesh + Q.mesh0;
   float ph = 0.f, e0 = 0.f, e1 = 0.f, e2 = 0.f;
+  if constexpr (IK) {
+    const long M = (long)nx * ny * nz;
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        const float *col = m0 + ((long)ix[a] * ny + iy[b]) * nz;
+        const float wxy = wx[a] * wy[b];
+#pragma unroll
+        for (int c = 0; c < P; c++) {
+          const float w = wxy * wz[c];
+          ph += w * col[iz[c]];
+          e0 += w * col[M + iz[c]];
+          e1 += w * col[2 * M + iz[c]];
+          e2 += w * col[3 * M + iz[c]];
+        }
+      }
+    }
+    kres[4 * si] = (double)ph;
+    kres[4 * si + 1] = (double)e0;
+    kres[4 * si + 2] = (double)e1;
+    kres[4 * si + 3] = (double)e2;
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < P; a++) {
 #pragma unroll
@@ -542,29 +621,38 @@ __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict_
 
 // forward z, y passes; x pass with the influence function; inverse y, z.
 // sp / nsg: the systems the grid's y extent covers (all of them, or one
-// system's parameters with nsg = 1)
+// system's parameters with nsg = 1); ik: some system forms its field on the
+// mesh (four components through the inverse passes)
+template <bool POW2, bool IK>
+static void launch_xgreen(Engine &e, int want_virial, int energies, hipStream_t st, dim3 g,
+                          const SysParams *sp) {
+  if (want_virial)
+    hipLaunchKernelGGL((k_fft_xgreen<true, POW2, true, IK>), g, dim3(FFT_TB), 0, st, sp,
+                       e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
+                       e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
+  else if (energies)
+    hipLaunchKernelGGL((k_fft_xgreen<false, POW2, true, IK>), g, dim3(FFT_TB), 0, st, sp,
+                       e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
+                       e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
+  else
+    hipLaunchKernelGGL((k_fft_xgreen<false, POW2, false, IK>), g, dim3(FFT_TB), 0, st, sp,
+                       e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys, e.d_acc,
+                       e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
+}
+
 template <bool POW2>
 static void launch_fft(Engine &e, int want_virial, int energies, hipStream_t st, int maxx,
-                       int maxy, int maxz, const SysParams *sp, int nsg) {
+                       int maxy, int maxz, const SysParams *sp, int nsg, bool ik = false) {
   hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, nsg), dim3(FFT_TB), 0, st, sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 0);
   hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, nsg), dim3(FFT_TB), 0, st, sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 1);
-  if (want_virial)
-    hipLaunchKernelGGL((k_fft_xgreen<true, POW2>), dim3(maxx, nsg), dim3(FFT_TB), 0, st,
-                       sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
-                       e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
-  else if (energies)
-    hipLaunchKernelGGL((k_fft_xgreen<false, POW2>), dim3(maxx, nsg), dim3(FFT_TB), 0, st,
-                       sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
-                       e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
-  else
-    hipLaunchKernelGGL((k_fft_xgreen<false, POW2, false>), dim3(maxx, nsg), dim3(FFT_TB), 0,
-                       st, sp, e.d_twid, e.d_green, e.d_cmesh, e.d_part, e.d_done + e.nsys,
-                       e.d_acc, e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
-  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, nsg), dim3(FFT_TB), 0, st, sp,
+  if (ik) launch_xgreen<POW2, true>(e, want_virial, energies, st, dim3(maxx, nsg), sp);
+  else launch_xgreen<POW2, false>(e, want_virial, energies, st, dim3(maxx, nsg), sp);
+  const int ncomp = ik ? 4 : 1;
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, nsg, ncomp), dim3(FFT_TB), 0, st, sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 3);
-  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, nsg), dim3(FFT_TB), 0, st, sp,
+  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, nsg, ncomp), dim3(FFT_TB), 0, st, sp,
                      e.d_twid, e.d_mesh, e.d_cmesh, 4);
 }
 
@@ -585,9 +673,15 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies) {
   hipLaunchKernelGGL(k_p3m_bin, gs, dim3(tb), 0, st, e.d_sp, e.d_r, e.d_q, e.nsite,
                      e.d_sgrid, e.d_sfrac, e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_err,
                      e.d_bin_fill + e.nbin_total);
-  bool used[MAXORDER + 1] = {false};
+  bool used[MAXORDER + 1] = {false}, used_ad[MAXORDER + 1] = {false},
+       used_ik[MAXORDER + 1] = {false};
+  bool ik = false;
   for (auto &p : e.hsp)
-    if (p.kmode == 1) used[p.order] = true;
+    if (p.kmode == 1) {
+      used[p.order] = true;
+      (p.ik ? used_ik : used_ad)[p.order] = true;
+      ik |= p.ik != 0;
+    }
 #define SPREAD(PO)                                                                          \
   if (used[PO])                                                                             \
     hipLaunchKernelGGL(k_p3m_spread<PO>, dim3(maxbins, e.nsys), dim3(256), 0, st,     \
@@ -600,11 +694,14 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies) {
   for (auto &p : e.hsp)
     if (p.kmode == 1)
       for (int n : {p.nx, p.ny, p.nz}) pow2 &= (n & (n - 1)) == 0;
-  if (pow2) launch_fft<true>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys);
-  else launch_fft<false>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys);
+  if (pow2) launch_fft<true>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys, ik);
+  else launch_fft<false>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys, ik);
 #define GATHER(PO)                                                                        \
-  if (used[PO])                                                                           \
+  if (used_ad[PO])                                                                        \
     hipLaunchKernelGGL(k_p3m_gather<PO>, gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid,     \
+                       e.d_sfrac, e.d_mesh, e.d_kres, e.d_bin_fill);                      \
+  if (used_ik[PO])                                                                        \
+    hipLaunchKernelGGL((k_p3m_gather<PO, true>), gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid, \
                        e.d_sfrac, e.d_mesh, e.d_kres, e.d_bin_fill);
   GATHER(1) GATHER(2) GATHER(3) GATHER(4) GATHER(5) GATHER(6) GATHER(7)
 #undef GATHER

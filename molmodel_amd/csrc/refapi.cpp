@@ -730,6 +730,7 @@ extern "C" p3m_t p3m_new(int nsite, const double ptv[9], double grid_spacing, in
   P.nsite = nsite;
   P.nx = p->ngrid[0]; P.ny = p->ngrid[1]; P.nz = p->ngrid[2];
   P.order = assignment_order;
+  P.ik = ik_differentiate ? 1 : 0;   // field on the mesh: three more meshes (pppm.c:444-455)
   if (P.nx < P.order || P.ny < P.order || P.nz < P.order)
     die_msg("p3m_new", "mesh dimensions below the assignment order are not supported on the device");
   const int nzh = P.nz / 2 + 1;
@@ -740,14 +741,15 @@ extern "C" p3m_t p3m_new(int nsite, const double ptv[9], double grid_spacing, in
   e.nbin_total = P.nbx * P.nby;   // d_bin_fill[nbin_total] records an overfull bin's need
   e.nblk_part = std::max(1, (P.ny * nzh + P3M_FFT_LINES - 1) / P3M_FFT_LINES);
   const size_t M = (size_t)P.nx * P.ny * P.nz, Mc = (size_t)P.nx * P.ny * nzh;
+  const size_t ncomp = P.ik ? 4 : 1;
   HIPCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
   HIPCHK(hipMalloc((void **)&e.d_sp, sizeof(SysParams)));
   HIPCHK(hipMalloc((void **)&e.d_r, sizeof(double) * 3 * N));
   HIPCHK(hipMalloc((void **)&e.d_q, sizeof(double) * N));
   HIPCHK(hipMalloc((void **)&e.d_kres, sizeof(double) * (4 * (size_t)N + 9)));
   HIPCHK(hipMalloc((void **)&e.d_acc, sizeof(double) * NACC));
-  HIPCHK(hipMalloc((void **)&e.d_mesh, sizeof(float) * M));
-  HIPCHK(hipMalloc((void **)&e.d_cmesh, sizeof(float2) * Mc));
+  HIPCHK(hipMalloc((void **)&e.d_mesh, sizeof(float) * M * ncomp));
+  HIPCHK(hipMalloc((void **)&e.d_cmesh, sizeof(float2) * Mc * ncomp));
   HIPCHK(hipMalloc((void **)&e.d_green, sizeof(float) * Mc));
   HIPCHK(hipMemset(e.d_green, 0, sizeof(float) * Mc));
   HIPCHK(hipMalloc((void **)&e.d_sgrid, sizeof(int4) * N));
@@ -779,8 +781,8 @@ extern "C" p3m_t p3m_new(int nsite, const double ptv[9], double grid_spacing, in
 // pppm.c:313-393: the influence function of an ngrid mesh over the cell
 // latv (columns are lattice vectors) at screening eta, into *green when
 // given (half spectrum [ix][iy][iz]), and the RMS force error estimate.
-double mmamd::p3m_influence(const double latv[9], const int ngrid[3], int nalias, int order, double eta,
-                     std::vector<double> *green) {
+double mmamd::p3m_influence(const double latv[9], const int ngrid[3], int nalias, int order, int ik,
+                            double eta, std::vector<double> *green) {
   const int nx = ngrid[0], ny = ngrid[1], nz = ngrid[2], nzh = nz / 2 + 1;
   double inv[9], rl[9];
   tensor_inverse(inv, latv);
@@ -797,7 +799,7 @@ double mmamd::p3m_influence(const double latv[9], const int ngrid[3], int nalias
     for (int iy = 0; iy < ny; iy++) {
       const int jy = iy > ny / 2 ? iy - ny : iy;
       for (int iz = 0; iz < nzh; iz++) {
-        const double g = green_at(jx, jy, iz, nx, ny, nz, rl, inv_4eta2, inv_v, nalias, order, &c);
+        const double g = green_at(jx, jy, iz, nx, ny, nz, rl, inv_4eta2, inv_v, nalias, order, ik, &c);
         if (green) (*green)[((size_t)ix * ny + iy) * nzh + iz] = g;
       }
     }
@@ -806,14 +808,13 @@ double mmamd::p3m_influence(const double latv[9], const int ngrid[3], int nalias
 }
 
 extern "C" void p3m_init(p3m_t p, double eta, double *chi) {   // pppm.c:313-393
-  *chi = p3m_influence(p->latv, p->ngrid, p->nalias, p->order, eta, &p->green);
+  *chi = p3m_influence(p->latv, p->ngrid, p->nalias, p->order, p->ik, eta, &p->green);
   p->eta = eta;
   p->have_green = 1;
   p3m_upload_params(p);
 }
 
 extern "C" void p3m_field(p3m_t p, const double *r, const double *q, double *phi, double *evec) {
-  if (p->ik) die_msg("p3m_field", "ik differentiation is not supported on the device");
   if (!p->have_green) die_msg("p3m_field", "p3m_init or p3m_optimize first");
   Engine &e = p->e;
   const int N = p->nsite;
@@ -876,9 +877,9 @@ extern "C" void p3m_delete(p3m_t p) {
 }
 
 // pppm.c:667-716: minimise chi_rspace^2 + chi_p3m^2 over eta
-void mmamd::p3m_optimize_eta(const double latv[9], const int ngrid[3], int nalias, int order,
-                      double grid_spacing, double rc, double *eta, double *chi_rspace,
-                      double *chi_p3m) {
+void mmamd::p3m_optimize_eta(const double latv[9], const int ngrid[3], int nalias, int order, int ik,
+                             double grid_spacing, double rc, double *eta, double *chi_rspace,
+                             double *chi_p3m) {
   const double *L = latv;
   const double v = fabs(-L[6] * L[4] * L[2] + L[3] * L[7] * L[2] + L[6] * L[1] * L[5] -
                         L[0] * L[7] * L[5] - L[3] * L[1] * L[8] + L[0] * L[4] * L[8]);
@@ -886,7 +887,7 @@ void mmamd::p3m_optimize_eta(const double latv[9], const int ngrid[3], int nalia
   auto f = [&](double x) {   // pppm.c:672-680
     if (x < 0) return 1e8;
     cr = 2 / sqrt(rc / pow(v, 1.0 / 3.0)) * exp(-(rc * x) * (rc * x));
-    ck = p3m_influence(latv, ngrid, nalias, order, x, nullptr);
+    ck = p3m_influence(latv, ngrid, nalias, order, ik, x, nullptr);
     return cr * cr + ck * ck;
   };
   // one-dimensional Nelder-Mead: start 0.1/h, step 0.1, size tolerance 1e-4
@@ -914,7 +915,7 @@ void mmamd::p3m_optimize_eta(const double latv[9], const int ngrid[3], int nalia
 }
 
 extern "C" void p3m_optimize(p3m_t p, double rc, double *eta, double *chi_rspace, double *chi_p3m) {
-  p3m_optimize_eta(p->latv, p->ngrid, p->nalias, p->order, p->grid_spacing, rc, eta, chi_rspace,
-                   chi_p3m);
+  p3m_optimize_eta(p->latv, p->ngrid, p->nalias, p->order, p->ik, p->grid_spacing, rc, eta,
+                   chi_rspace, chi_p3m);
   p3m_init(p, *eta, chi_p3m);   // leave the influence function at the returned eta
 }

@@ -1,0 +1,204 @@
+"""P3M with the field formed as -i k phi on the mesh (p3m_ik_differentiate:
+influence function pppm.c:370-383, field pppm.c:577-599, interpolation
+pppm.c:618-626) on the device, against the FP64 oracle's ik mode on the same
+seeded inputs.
+
+Bars, as for real-space differentiation: fields and forces <= 1e-5 relative
+(FP32 mesh and FFTs, FP64 accumulation), energies <= 1e-5 of |E_total|,
+virial <= 1e-5 relative; the error estimate of p3m_init (FP64 on the host)
+to 1e-10.  The ik field differs from the real-space differentiated one by
+more than the bar on these inputs (2.7e-5 and 3.5e-4), so a device that
+ignored the flag fails.  Momentum: sum_i q_i E_i vanishes identically for
+ik (it does not for real-space differentiation); the device's FP32 mesh
+leaves rounding only.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from molmodel_amd import engine as E
+from molmodel_amd import waterbox as W
+from molmodel_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+P = C.c_void_p
+D = C.c_double
+I = C.c_int
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def momentum(q, ev):
+    F = q[:, None] * ev
+    return float(np.abs(F.sum(0)).max() / np.abs(F).sum())
+
+
+def small_box(ik, seed=31):
+    """1536 sites, L 24.86, mesh 24^3 (not a power of two), order 7"""
+    s = W.water_box(8, 24.86, seed)
+    p = dict(rc=10.0, smoothing_width=1.0, eta=0.33, kmode=1, mesh=(24, 24, 24), order=7,
+             skin=W.default_skin(10.0, 24.86), nalias=2, ik=ik, include_lj_correction=1)
+    return s, p
+
+
+@pytest.fixture(scope="module")
+def box12k():
+    s, p = W.config("water12k_p3m")
+    return s, dict(p, ik=1)
+
+
+def test_ik_field_matches_oracle(gpu_lib, box12k):
+    s, p = box12k
+    print("[ik] field water12k mesh 48 order 5", flush=True)
+    e = Engine(s, p)
+    e.set_positions(s.r)
+    phi, ev = e.kspace_field()
+    ophi, oev, _, _ = O.p3m_parts(s, p)
+    _, aev, _, _ = O.p3m_parts(s, dict(p, ik=0))
+    m_dev, m_ad = momentum(s.q, ev), momentum(s.q, aev)
+    print(f"[ik] phi {rel(phi, ophi):.3e} E {rel(ev, oev):.3e}; ik vs real-space "
+          f"differentiation {rel(aev, oev):.3e}; momentum device {m_dev:.3e}, "
+          f"real-space differentiation {m_ad:.3e}")
+    assert rel(phi, ophi) < 1e-5 and rel(ev, oev) < 1e-5
+    assert rel(aev, oev) > 1e-5
+    assert m_dev < 0.1 * m_ad
+
+
+@pytest.mark.parametrize("which", ["water12k", "small"])
+def test_ik_forces_energy_virial(gpu_lib, box12k, which):
+    s, p = box12k if which == "water12k" else small_box(1)
+    print(f"[ik] forces {which} N={s.nsite}", flush=True)
+    e = Engine(s, p)
+    e.set_positions(s.r)
+    e.compute(virial=True)
+    f, en, w = e.forces(), e.energies()[0], e.virial()[0]
+    of, oen, ow = O.evaluate(s, p, virial=True)
+    print(f"[ik] {which}: force rel {rel(f, of):.3e} elec {en[0]:.8f} vs {oen[0]:.8f} "
+          f"virial rel {rel(w, ow.reshape(3, 3)):.3e}")
+    assert rel(f, of) <= 1e-5
+    assert abs(en[0] - oen[0]) <= 1e-5 * abs(oen[3])
+    assert abs(en[1] - oen[1]) <= 1e-5 * abs(oen[3])
+    assert rel(w, ow.reshape(3, 3)) <= 1e-5
+
+
+def test_ik_batch_with_real_space_differentiation(gpu_lib):
+    """One engine holding both kinds (the same box with and without ik, and
+    a 30^3 ik mesh of order 5) gives every system what an engine of its own
+    gives it, bit for bit."""
+    mem = [small_box(0), small_box(1, seed=32)]
+    s = W.water_box(10, 31.07, 33)
+    mem.append((s, dict(rc=10.0, smoothing_width=1.0, eta=0.32, kmode=1, mesh=(30, 30, 30),
+                        order=5, skin=W.default_skin(10.0, 31.07), nalias=2, ik=1,
+                        include_lj_correction=1)))
+    print("[ik] batch of", [(m[0].nsite, m[1]["ik"]) for m in mem], flush=True)
+    e = Engine([m[0] for m in mem], [m[1] for m in mem])
+    e.set_positions(np.concatenate([m[0].r for m in mem]))
+    e.compute()
+    fb, eb, off = e.forces(), e.energies(), e.offsets
+    for k, (s, p) in enumerate(mem):
+        one = Engine(s, p)
+        one.set_positions(s.r)
+        one.compute()
+        assert np.array_equal(fb[off[k]:off[k + 1]], one.forces()), k
+        assert np.array_equal(eb[k][:4], one.energies()[0][:4]), k
+
+
+def test_ik_md_then_forces(gpu_lib):
+    """A graph-captured MD block (forces-only inner steps through the ik
+    kernels) leaves a state whose forces match the oracle's at the new
+    positions."""
+    s, p = small_box(1)
+    e = Engine(s, p)
+    e.set_positions(s.r)
+    e.set_velocities(s.v)
+    e.md_setup(dt=0.001)
+    print("[ik] md 30 steps", flush=True)
+    e.md_run(30)
+    r = e.positions()
+    assert np.isfinite(r).all() and not np.array_equal(r, s.r)
+    e.compute()
+    t = W.System(**{k: getattr(s, k) for k in s.__dataclass_fields__})
+    t.r = np.ascontiguousarray(r)
+    of, _ = O.evaluate(t, p)
+    print(f"[ik] after md: force rel {rel(e.forces(), of):.3e}")
+    assert rel(e.forces(), of) <= 1e-5
+
+
+def test_ik_adjoint_refused_basis_works(gpu_lib):
+    """The adjoint kernels are built for real-space differentiation; an ik
+    system is refused with a message and the basis of forward passes
+    serves it."""
+    s, p = small_box(1)
+    e = Engine(s, p)
+    e.set_positions(s.r)
+    with pytest.raises(E.MMError, match="ik"):
+        e.param_gradients(0)
+    g = e.param_gradients(0, basis=True)
+    assert g["ntype"] >= 2 and np.isfinite(g["dE"]).all()
+
+
+def lib():
+    L = E.lib()
+    L.p3m_new.restype = P
+    L.p3m_new.argtypes = [I, P, D, I, I, I, P]
+    L.p3m_init.argtypes = [P, D, P]
+    L.p3m_optimize.argtypes = [P, D, P, P, P]
+    L.p3m_field.argtypes = [P, P, P, P, P]
+    L.p3m_delete.argtypes = [P]
+    return L
+
+
+def test_ik_p3m_dropin(gpu_lib, box12k):
+    """p3m_new(..., ik_differentiate = 1, ...), p3m_init, p3m_field as
+    ElecLJ calls them (eleclj.cpp:201-210, 932-950): the influence
+    function's error estimate is the oracle's ik one, the field the oracle's
+    ik field; with evec NULL the potential alone."""
+    L = lib()
+    s, p = box12k
+    latv = np.array([s.boxl, 0, 0, 0, s.boxl, 0, 0, 0, s.boxl])
+    ng = np.zeros(3, dtype=np.int32)
+    pp = L.p3m_new(s.nsite, latv.ctypes.data_as(P), s.boxl / 48, 2, 1, 5, ng.ctypes.data_as(P))
+    assert list(ng) == [48, 48, 48]
+    chi = np.zeros(1)
+    L.p3m_init(pp, p["eta"], chi.ctypes.data_as(P))
+    _, ochi = O.green((48, 48, 48), s.boxl, p["eta"], 5, ik=1)
+    _, achi = O.green((48, 48, 48), s.boxl, p["eta"], 5, ik=0)
+    assert abs(chi[0] - ochi) < 1e-10 * ochi and abs(achi - ochi) > 1e-3 * ochi
+    phi, ev = np.zeros(s.nsite), np.zeros((s.nsite, 3))
+    L.p3m_field(pp, s.r.ctypes.data_as(P), s.q.ctypes.data_as(P), phi.ctypes.data_as(P),
+                ev.ctypes.data_as(P))
+    phi0 = np.zeros(s.nsite)
+    L.p3m_field(pp, s.r.ctypes.data_as(P), s.q.ctypes.data_as(P), phi0.ctypes.data_as(P), None)
+    L.p3m_delete(pp)
+    ophi, oev, _, _ = O.p3m_parts(s, p)
+    print(f"[ik] drop-in phi {rel(phi, ophi):.3e} E {rel(ev, oev):.3e}")
+    assert rel(phi, ophi) < 1e-5 and rel(ev, oev) < 1e-5
+    assert np.array_equal(phi, phi0)
+
+
+def test_ik_p3m_optimize_uses_the_ik_estimate(gpu_lib):
+    """p3m_optimize on an ik handle minimises chi_r^2 + chi_ik^2: its
+    estimates are those of the oracle's ik influence function at the
+    returned eta, and the optimum differs from the real-space
+    differentiated one."""
+    L = lib()
+    s, _ = W.config("water648_ewald")
+    rc, mesh = 9.0, 20
+    latv = np.array([s.boxl, 0, 0, 0, s.boxl, 0, 0, 0, s.boxl])
+    ng = np.zeros(3, dtype=np.int32)
+    out = {}
+    for ik in (0, 1):
+        pp = L.p3m_new(s.nsite, latv.ctypes.data_as(P), s.boxl / mesh, 2, ik, 5, ng.ctypes.data_as(P))
+        eta, cr, ck = np.zeros(1), np.zeros(1), np.zeros(1)
+        L.p3m_optimize(pp, rc, eta.ctypes.data_as(P), cr.ctypes.data_as(P), ck.ctypes.data_as(P))
+        L.p3m_delete(pp)
+        _, oc = O.green((mesh,) * 3, s.boxl, eta[0], 5, ik=ik)
+        assert abs(ck[0] - oc) <= 1e-9 * oc
+        out[ik] = eta[0]
+    print(f"[ik] optimal eta: real-space differentiation {out[0]:.5f}, ik {out[1]:.5f}")
+    assert abs(out[0] - out[1]) > 1e-3 * out[0]

@@ -12,6 +12,13 @@ formula:
     under the homogeneous strain x_a -> x_a + e x_b of positions and lattice
     (the influence function recomputed for each strained lattice), so the
     diagonal and off-diagonal components are pinned one by one;
+  * the P3M field formed as -i k phi on the mesh (pppm.c:577-599): momentum
+    conservation (sum q E = 0, which real-space differentiation does not
+    have), convergence of each field component to the Ewald reciprocal
+    field, and the influence function's error estimate against the
+    measured RMS force error of uncorrelated charges (also for real-space
+    differentiation), plus the alias-free limit where both influence
+    functions are g / U^2;
   * the real-space virial (pair, switching and excluded-pair terms) against
     sum_i r_i (x) F_i of the oracle's own real-space forces for a cluster
     that no periodic image reaches, where the two are equal identically;
@@ -108,15 +115,15 @@ def kspace_energy(s, A, r, eta, kc):
     return en[0], vir.reshape(3, 3)
 
 
-def p3m_energy(s, A, r, eta, mesh, order):
+def p3m_energy(s, A, r, eta, mesh, order, ik=0):
     nx, ny, nz = mesh
     latv = tensor_layout(A)
     gh = np.zeros(nx * ny * (nz // 2 + 1))
     lib = O.oracle()
-    lib.orc_p3m_green(nx, ny, nz, O._ptr(latv), eta, 2, order, 0, O._ptr(gh))
+    lib.orc_p3m_green(nx, ny, nz, O._ptr(latv), eta, 2, order, ik, O._ptr(gh))
     n = s.nsite
     phi = np.zeros(n); ev = np.zeros((n, 3)); vir = np.zeros(9); en = np.zeros(1)
-    lib.orc_p3m_field(n, O._ptr(r), O._ptr(s.q), nx, ny, nz, O._ptr(latv), eta, order, 0,
+    lib.orc_p3m_field(n, O._ptr(r), O._ptr(s.q), nx, ny, nz, O._ptr(latv), eta, order, ik,
                       O._ptr(gh), O._ptr(phi), O._ptr(ev), O._ptr(vir), O._ptr(en))
     return en[0], vir.reshape(3, 3)
 
@@ -150,14 +157,105 @@ def test_kspace_virial_components_equal_strain_derivatives():
 # as the optimal influence function follows the continuum one; the bar is
 # the measured agreement with a margin (a dropped or transposed term moves
 # components by 1e-2 and more of the largest).
-@pytest.mark.parametrize("mesh,order,tol", [((24, 24, 24), 5, 2e-7), ((20, 24, 18), 4, 1e-5)])
-def test_p3m_virial_components_equal_strain_derivatives(mesh, order, tol):
+@pytest.mark.parametrize("mesh,order,ik,tol", [((24, 24, 24), 5, 0, 2e-7), ((20, 24, 18), 4, 0, 1e-5),
+                                               ((24, 24, 24), 5, 1, 1e-8), ((20, 24, 18), 4, 1, 1e-8)])
+def test_p3m_virial_components_equal_strain_derivatives(mesh, order, ik, tol):
     s, _ = W.config("water216_small")
     eta = 0.35
-    _, w = p3m_energy(s, np.eye(3) * s.boxl, np.ascontiguousarray(s.r), eta, mesh, order)
-    fd = strain_derivatives(lambda A, r: p3m_energy(s, A, r, eta, mesh, order)[0], s)
+    _, w = p3m_energy(s, np.eye(3) * s.boxl, np.ascontiguousarray(s.r), eta, mesh, order, ik)
+    fd = strain_derivatives(lambda A, r: p3m_energy(s, A, r, eta, mesh, order, ik)[0], s)
     print("P3M virial\n", w, "\nstrain derivatives\n", fd)
     assert np.abs(fd - w).max() <= tol * np.abs(w).max()
+
+
+# ---------------------------------------------------------------- P3M field by -i k
+def random_charges(n=300, boxl=20.0, seed=5):
+    """n unit charges of alternating sign at uniform random positions: the
+    uncorrelated charges the P3M error estimate is derived for"""
+    rng = np.random.default_rng(seed)
+    r = np.ascontiguousarray(rng.uniform(0, boxl, (n, 3)))
+    q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    return Charges(r, q, boxl)
+
+
+class Charges:
+    """positions and charges in a cubic box (what the oracle's k-space and
+    P3M fields read of a system)"""
+    def __init__(self, r, q, boxl):
+        self.r, self.q, self.boxl, self.nsite = r, q, boxl, len(q)
+
+
+def p3m_of(s, mesh, order, ik, eta=0.35, nalias=2):
+    return O.p3m_parts(s, dict(mesh=(mesh,) * 3, order=order, eta=eta, nalias=nalias, ik=ik))
+
+
+@pytest.mark.parametrize("which", ["water27", "random"])
+def test_p3m_ik_conserves_momentum(which):
+    """Forming the field as -i k G rho_k on the mesh and interpolating it with
+    the assignment weights gives sum_i q_i E_i = sum_k (-i k) G |rho_k|^2,
+    which vanishes term by term against -k (G and |rho_k|^2 are even in k),
+    for any configuration.  Real-space differentiation has no such identity:
+    its sum is far from zero on the same input, so the check can fail."""
+    s = W.config("water27_small")[0] if which == "water27" else random_charges()
+    for ik, lo, hi in ((1, 0.0, 1e-13), (0, 1e-9, 1.0)):
+        _, ev, _, _ = p3m_of(s, 16, 5, ik)
+        F = s.q[:, None] * ev
+        mom = np.abs(F.sum(0)).max() / np.abs(F).sum()
+        print(f"[ik] {which} ik={ik} momentum {mom:.3e}")
+        assert lo <= mom <= hi
+
+
+def test_p3m_ik_field_converges_to_ewald():
+    """As the mesh refines, each component of the ik field approaches the
+    Ewald reciprocal-space field of the same screening (the oracle's direct
+    sum over wave vectors to kc = 12); a swapped, negated or mis-scaled
+    component stays at O(1)."""
+    s, _ = W.config("water27_small")
+    eta = 0.35
+    _, kev, _, _ = O.kspace_parts(s, 12.0, eta)
+    errs = []
+    for mesh in (12, 24, 32):
+        _, ev, _, _ = p3m_of(s, mesh, 5, 1, eta)
+        e = [float(np.linalg.norm(ev[:, c] - kev[:, c]) / np.linalg.norm(kev[:, c])) for c in range(3)]
+        print(f"[ik] mesh {mesh} component errors {e}")
+        errs.append(max(e))
+    assert errs[0] < 1e-3 and errs[2] < 2e-6 and errs[2] < errs[1] < errs[0]
+
+
+@pytest.mark.parametrize("ik", [0, 1])
+def test_p3m_error_estimate_predicts_measured_error(ik):
+    """The influence function's error estimate chi (pppm.c:386-392) predicts
+    the RMS force error of uncorrelated charges against Ewald,
+    dF = chi sum q^2 / (sqrt(N) L^2), across meshes and orders that move dF
+    over four decades, for either differentiation; a dropped alias term or a
+    wrong denominator in chi breaks the proportionality."""
+    s = random_charges()
+    eta = 0.4
+    _, kev, _, _ = O.kspace_parts(s, 16.0, eta)
+    F = s.q[:, None] * kev
+    ratios = []
+    for mesh in (12, 20, 32):
+        for order in (3, 5):
+            _, ev, _, _ = p3m_of(s, mesh, order, ik, eta)
+            rms = math.sqrt(((s.q[:, None] * ev - F) ** 2).sum() / s.nsite)
+            _, chi = O.green((mesh,) * 3, s.boxl, eta, order, 2, ik)
+            ratios.append(rms / (chi * (s.q ** 2).sum() / (math.sqrt(s.nsite) * s.boxl ** 2)))
+    print(f"[ik] ik={ik} measured / estimated error", np.round(ratios, 3))
+    assert 0.75 <= min(ratios) and max(ratios) <= 1.25
+
+
+def test_p3m_ik_influence_without_aliasing():
+    """With no alias terms the optimal influence function of either
+    differentiation is g(k) / U^2(k) (pppm.c:370-383 with one term), so the
+    two agree to rounding and the error estimate vanishes; with aliasing the
+    ik estimate is the smaller (its measured error is too)."""
+    g0, c0 = O.green((16, 16, 16), 9.32, 0.35, 5, nalias=0, ik=0)
+    g1, c1 = O.green((16, 16, 16), 9.32, 0.35, 5, nalias=0, ik=1)
+    assert np.abs(g0 - g1).max() <= 1e-14 * np.abs(g0).max()
+    assert c0 < 1e-6 and c1 < 1e-6
+    _, a0 = O.green((16, 16, 16), 9.32, 0.35, 5, nalias=2, ik=0)
+    _, a1 = O.green((16, 16, 16), 9.32, 0.35, 5, nalias=2, ik=1)
+    assert 1e-6 < a1 < a0
 
 
 # ---------------------------------------------------------------- real-space virial
