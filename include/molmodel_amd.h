@@ -74,7 +74,7 @@ typedef struct {
   int mesh[3];               /* P3M mesh */
   int order;                 /* p3m_assignment_order (1..7) */
   int nalias;                /* p3m_alias_sum_limit */
-  int ik;                    /* p3m_ik_differentiate (0 only on the device) */
+  int ik;                    /* p3m_ik_differentiate: field as -i k phi on the mesh */
   int include_lj_correction;
   int geometric_sigma;       /* geometric_combining_for_sigma */
 } mm_system_desc;
@@ -165,8 +165,8 @@ int mm_msim_eleclj_table(const mm_msim_eleclj *p, const char *table, const char 
  * include_lj_correction, geometric_combining_for_sigma, then
  * mm_desc_resolve with p3m_grid_spacing / p3m_grid_density.  Parameters
  * the device path does not evaluate (virtual sites, bond charge
- * increments, 1-3 interactions, 1-4 scaling, ik differentiation, a term
- * switched off) are errors.  Sites and groups are left to the caller. */
+ * increments, 1-3 interactions, 1-4 scaling, a term switched off) are
+ * errors; p3m_ik_differentiate sets ik.  Sites and groups are left to the caller. */
 int mm_msim_eleclj_desc(const mm_msim_eleclj *p, mm_system_desc *d);
 /* per-atom charge (sqrt(kcal A/mol)), sigma, epsilon and site mask from the
  * atoms' types as ElecLJ::types_changed derives them (eleclj.cpp:440-500):

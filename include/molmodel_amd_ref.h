@@ -10,8 +10,8 @@
  *
  * Conditions the reference asserts on abort here too, with a message;
  * inputs the device path does not cover (non-cubic real space, screening
- * radii, 1-4 scaling, exclusions between groups, ik P3M differentiation)
- * also abort with a message rather than being computed any other way.
+ * radii, 1-4 scaling, exclusions between groups) also abort with a
+ * message rather than being computed any other way.
  */
 #ifndef MOLMODEL_AMD_REF_H
 #define MOLMODEL_AMD_REF_H
@@ -161,6 +161,10 @@ int kspace_field_noalloc(double cutoff, int nsite, const cart_t *r, const double
                          cart_t *evec, void *stream);
 
 /* ---- pppm.h:8-64 ---- */
+/* ik_differentiate 1: the field is formed as -i k phi on the mesh and
+ * interpolated with the assignment weights (three more meshes and inverse
+ * transforms, pppm.c:577-626); 0: analytical differentiation of the
+ * weights (pppm.c:603-640). */
 typedef struct p3m *p3m_t;
 p3m_t p3m_new(int nsite, const double primitive_translation_vectors[9], double grid_spacing,
               int nalias, int ik_differentiate, int assignment_order, int ngrid[3]);

@@ -572,10 +572,6 @@ int mm_msim_eleclj_desc(const mm_msim_eleclj *p, mm_system_desc *d) {
     set_last_error("ElecLJ: 1-4 scale factors other than 1 are not evaluated on the device");
     return MM_ERR_ARG;
   }
-  if (p->flag.at("p3m_ik_differentiate")) {
-    set_last_error("ElecLJ: p3m_ik_differentiate: the device P3M differentiates analytically");
-    return MM_ERR_ARG;
-  }
   const double boxl = d->boxl;
   mm_desc_init(d);
   d->boxl = boxl;
@@ -587,7 +583,7 @@ int mm_msim_eleclj_desc(const mm_msim_eleclj *p, mm_system_desc *d) {
   d->kmode = p->flag.at("use_p3m") ? 1 : 0;
   d->order = p->num.at("p3m_assignment_order");
   d->nalias = p->num.at("p3m_alias_sum_limit");
-  d->ik = 0;
+  d->ik = p->flag.at("p3m_ik_differentiate") ? 1 : 0;
   d->include_lj_correction = p->flag.at("include_lj_correction");
   d->geometric_sigma = p->flag.at("geometric_combining_for_sigma");
   return mm_desc_resolve(d, p->dbl.at("p3m_grid_spacing"), p->dbl.at("p3m_grid_density"));

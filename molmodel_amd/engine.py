@@ -192,8 +192,8 @@ def tune_kspace(boxl, nsite, rc, tol=1e-4, allow_ewald=True):
 def eleclj_defaults(boxl, p3m=False, p3m_grid_spacing=0.5, p3m_grid_density=-1.0, **given):
     """ElecLJ's parameter defaults for a cubic box (mm_desc_init, then
     mm_desc_resolve): pass any of rc, skin, smoothing_width, eta, kc, order,
-    nalias, mesh to fix them; returns the resolved parameters as a dict of
-    the waterbox params keys.  Host only."""
+    nalias, ik, mesh to fix them; returns the resolved parameters as a dict
+    of the waterbox params keys.  Host only."""
     d = SystemDesc()
     lib().mm_desc_init(C.byref(d))
     d.boxl = float(boxl)
@@ -201,7 +201,7 @@ def eleclj_defaults(boxl, p3m=False, p3m_grid_spacing=0.5, p3m_grid_density=-1.0
     for k in ("rc", "skin", "smoothing_width", "eta", "kc"):
         if k in given:
             setattr(d, k, float(given[k]))
-    for k in ("order", "nalias"):
+    for k in ("order", "nalias", "ik"):
         if k in given:
             setattr(d, k, int(given[k]))
     if "mesh" in given:
@@ -270,14 +270,14 @@ class MsimElecLJ:
         return None if rc == 1 else v.value
 
     def desc(self, boxl):
-        """(rc, smoothing_width, skin, eta, kmode, kc, mesh, order, nalias,
+        """(rc, smoothing_width, skin, eta, kmode, kc, mesh, order, nalias, ik,
         include_lj_correction, geometric_sigma) after mm_msim_eleclj_desc."""
         d = SystemDesc()
         d.boxl = boxl
         _check(lib().mm_msim_eleclj_desc(self.h, C.byref(d)))
         return dict(rc=d.rc, smoothing_width=d.smoothing_width, skin=d.skin, eta=d.eta,
                     kmode=d.kmode, kc=d.kc, mesh=tuple(d.mesh), order=d.order, nalias=d.nalias,
-                    include_lj_correction=d.include_lj_correction,
+                    ik=d.ik, include_lj_correction=d.include_lj_correction,
                     geometric_sigma=d.geometric_sigma)
 
     def site_params(self, types):

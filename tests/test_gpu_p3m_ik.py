@@ -202,3 +202,57 @@ def test_ik_p3m_optimize_uses_the_ik_estimate(gpu_lib):
         out[ik] = eta[0]
     print(f"[ik] optimal eta: real-space differentiation {out[0]:.5f}, ik {out[1]:.5f}")
     assert abs(out[0] - out[1]) > 1e-3 * out[0]
+
+
+def example_inputs():
+    """The reference's P3M example (pppm-example.c:8-13, 60-76): five
+    charges in (-1, 1) and positions in (-1, 1)^3 from its linear
+    congruential generator, seed 314159265, for a unit cube."""
+    seed = 314159265
+
+    def rnd():
+        nonlocal seed
+        seed = (1103515245 * seed + 12345) & 0xFFFFFFFF
+        return ((seed // 65536) % 32768) / 32769.0
+
+    q = np.array([2 * rnd() - 1 for _ in range(5)])
+    r = np.array([[2 * rnd() - 1 for _ in range(3)] for _ in range(5)])
+    return q, r
+
+
+def test_reference_example_through_the_dropins(gpu_lib):
+    """pppm-example.c end to end: p3m_new(5 sites, unit cube, spacing 0.1,
+    nalias 2, ik, order 3), p3m_optimize(rc 0.4), p3m_field.  The example
+    prints its results without expected values; here the mesh is 10^3, the
+    field matches the oracle's ik field at the returned eta, and both match
+    the Ewald reciprocal-space field to within three times the error that
+    the returned estimate predicts (positions outside the cell exercise the
+    wrap)."""
+    L = lib()
+    q, r = example_inputs()
+    ptv = np.eye(3).reshape(-1)
+    ng = np.zeros(3, dtype=np.int32)
+    pp = L.p3m_new(5, ptv.ctypes.data_as(P), 0.1, 2, 1, 3, ng.ctypes.data_as(P))
+    assert list(ng) == [10, 10, 10]
+    eta, cr, ck = np.array([5.6]), np.zeros(1), np.zeros(1)
+    L.p3m_optimize(pp, 0.4, eta.ctypes.data_as(P), cr.ctypes.data_as(P), ck.ctypes.data_as(P))
+    phi, ev = np.zeros(5), np.zeros((5, 3))
+    r = np.ascontiguousarray(r)
+    L.p3m_field(pp, r.ctypes.data_as(P), q.ctypes.data_as(P), phi.ctypes.data_as(P),
+                ev.ctypes.data_as(P))
+    L.p3m_delete(pp)
+
+    class Sites:
+        boxl, nsite = 1.0, 5
+    s = Sites()
+    s.r, s.q = r, q
+    ophi, oev, _, _ = O.p3m_parts(s, dict(mesh=(10, 10, 10), eta=eta[0], order=3, nalias=2, ik=1))
+    _, kev, _, _ = O.kspace_parts(s, 14.0, eta[0])
+    dF = np.sqrt(((q[:, None] * (oev - kev)) ** 2).sum() / 5)
+    predicted = ck[0] * (q ** 2).sum() / np.sqrt(5)
+    print(f"[ik] example: eta {eta[0]:.4f} chi_r {cr[0]:.3e} chi_k {ck[0]:.3e}; "
+          f"phi {rel(phi, ophi):.3e} E {rel(ev, oev):.3e}; vs Ewald {dF:.3e} (predicted {predicted:.3e})")
+    for i in range(5):
+        print(f"[ik]   site {i}: q {q[i]: .6f} phi {phi[i]: .6f} E {ev[i, 0]: .6f} {ev[i, 1]: .6f} {ev[i, 2]: .6f}")
+    assert rel(phi, ophi) < 1e-5 and rel(ev, oev) < 1e-5
+    assert dF < 3 * predicted

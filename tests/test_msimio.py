@@ -204,11 +204,23 @@ def test_descriptor_matches_the_defaults_path():
     ("{ one_three_interaction bci.tab }", "'one_three_interaction'"),
     ("{ include_kspace no }", "include_kspace"),
     ("{ elec_1_4_scale_factor 0.5 }", "1-4"),
-    ("{ p3m_ik_differentiate yes }", "ik"),
 ])
 def test_descriptor_rejects_what_the_device_does_not_evaluate(text, msg):
     with pytest.raises(E.MMError, match=msg):
         E.MsimElecLJ(text).desc(20.0)
+
+
+def test_ik_differentiation_reaches_the_descriptor():
+    """p3m_ik_differentiate yes sets ik, and the screening chosen for it
+    minimises the error estimate of the ik influence function (as
+    ElecLJ::init's p3m_optimize does on an ik handle), which differs from
+    the real-space differentiated choice."""
+    text = "{ use_p3m yes p3m_ik_differentiate %s rspace_cutoff 9 p3m_grid_spacing 1.0 }"
+    d1 = E.MsimElecLJ(text % "yes").desc(24.86)
+    d0 = E.MsimElecLJ(text % "no").desc(24.86)
+    assert d1["ik"] == 1 and d0["ik"] == 0 and d1["mesh"] == d0["mesh"]
+    ref = E.eleclj_defaults(24.86, p3m=True, p3m_grid_spacing=1.0, rc=9.0, ik=1)
+    assert d1["eta"] == ref["eta"] and d1["eta"] != d0["eta"]
 
 
 def test_empty_unsupported_tables_are_fine():
