@@ -228,12 +228,18 @@ int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out);
  *   spread_block  P3M spreading block side in mesh columns (the smallest
  *                 factor of the mesh side >= it); mesh bit-identical;
  *   graph_steps   MD steps captured per graph (1..64); trajectories
- *                 bit-identical.
- * mm_shape_default: 2, 3, 2, 10 (MOLMODEL_AMD_PAIR_SLOTS,
- * MOLMODEL_AMD_SPREAD_B and MOLMODEL_AMD_GRAPH_STEPS override the last
- * three); mm_engine_create uses it. */
+ *                 bit-identical;
+ *   spread_halo   P3M spreading by owner (1: each block spreads the sites
+ *                 of its own columns whole into a halo that the z pass
+ *                 sums; pays where columns hold few sites) or by pulling
+ *                 (0: each block gathers the sites whose stencils reach its
+ *                 columns); mesh bit-identical.
+ * mm_shape_default: 2, 3, 2, 10, 0 (MOLMODEL_AMD_PAIR_SLOTS,
+ * MOLMODEL_AMD_SPREAD_B, MOLMODEL_AMD_GRAPH_STEPS and
+ * MOLMODEL_AMD_SPREAD_HALO override the last four); mm_engine_create uses
+ * it. */
 typedef struct {
-  int cell_groups, pair_slots, spread_block, graph_steps;
+  int cell_groups, pair_slots, spread_block, graph_steps, spread_halo;
 } mm_shape;
 void mm_shape_default(mm_shape *s);
 int mm_engine_create_shaped(int nsys, const mm_system_desc *sys, const mm_shape *shape,
@@ -243,8 +249,9 @@ int mm_engine_create_shaped(int nsys, const mm_system_desc *sys, const mm_shape 
  * caller's batch (positions r, velocities v or NULL), runs MD with
  * mm_md_setup(dt, bond, angle_deg, tol), and times `steps`-step blocks
  * (graph replay; the best of two after a warm-up); one choice at a time
- * from the default shape: pair_slots {2, 3, 4}, spread_block {2, 4} (when a
- * system uses P3M), cell_groups {1, 2, 4}, graph_steps {5, 10, 20}.  *best:
+ * from the default shape: pair_slots {2, 3, 4}, spread_block {2, 4} and
+ * spread_halo {0, 1} (when a system uses P3M), cell_groups {1, 2, 4},
+ * graph_steps {5, 10, 20}.  *best:
  * the fastest; trials[0..max_trials) the shapes timed and their ms per
  * step (-1 for a shape that could not run), *ntrials their number.  The
  * caller's own engines are not touched. */

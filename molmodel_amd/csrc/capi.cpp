@@ -87,16 +87,19 @@ void mm_shape_default(mm_shape *s) {
   s->pair_slots = 3;
   s->spread_block = 2;
   s->graph_steps = 10;
+  s->spread_halo = 0;
   if (const char *v = getenv("MOLMODEL_AMD_PAIR_SLOTS")) s->pair_slots = atoi(v);
+  if (const char *v = getenv("MOLMODEL_AMD_SPREAD_HALO")) s->spread_halo = atoi(v);
   if (const char *v = getenv("MOLMODEL_AMD_SPREAD_B")) s->spread_block = atoi(v);
   if (const char *v = getenv("MOLMODEL_AMD_GRAPH_STEPS")) s->graph_steps = atoi(v);
 }
 
 static int check_shape(const mm_shape &s) {
   if (s.cell_groups < 1 || s.cell_groups > 8 || s.pair_slots < 1 || s.pair_slots > 4 ||
-      s.spread_block < 1 || s.spread_block > 8 || s.graph_steps < 1 || s.graph_steps > 64) {
+      s.spread_block < 1 || s.spread_block > 8 || s.graph_steps < 1 || s.graph_steps > 64 ||
+      s.spread_halo < 0 || s.spread_halo > 1) {
     set_last_error("mm_shape: cell_groups 1..8, pair_slots 1..4, spread_block 1..8, "
-                   "graph_steps 1..64");
+                   "graph_steps 1..64, spread_halo 0 or 1");
     return MM_ERR_ARG;
   }
   return MM_OK;
@@ -127,6 +130,7 @@ int mm_engine_create_shaped(int nsys, const mm_system_desc *sys, const mm_shape 
   m->e.pair_slots = shape->pair_slots;
   m->e.spread_block = shape->spread_block;
   m->e.graph_steps = shape->graph_steps;
+  m->e.spread_halo = shape->spread_halo;
   int rc = engine_create(m->e, in);
   if (rc) {
     engine_destroy(m->e);
@@ -150,6 +154,7 @@ int mm_check_layout(int nsys, const mm_system_desc *sys) {
   TRY(check_shape(s));
   e.cell_groups = s.cell_groups;
   e.spread_block = s.spread_block;
+  e.spread_halo = s.spread_halo;
   HostLayout h;
   if (engine_layout(e, in, h)) return MM_ERR_ARG;
   if (engine_check_layout(e, h)) return MM_ERR_ARG;

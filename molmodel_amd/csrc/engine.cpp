@@ -404,7 +404,7 @@ int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h) 
   std::vector<float> &green = h.green;
   std::vector<float2> &twid = h.twid;
   int site0 = 0, group0 = 0, cell0 = 0, k0 = 0, bin0 = 0;
-  long mesh0 = 0, cmesh0 = 0, green0 = 0;
+  long mesh0 = 0, cmesh0 = 0, green0 = 0, halo0 = 0;
   int maxnb = 64;
   for (int s = 0; s < e.nsys; s++) {
     const SystemInput &S = in[s];
@@ -564,12 +564,16 @@ int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h) 
           const double a = -2 * M_PI * (t % nn[d]) / nn[d];
           twid.push_back(make_float2((float)cos(a), (float)sin(a)));
         }
-      MM_REQ(choose_spread_blocks(S.nx, S.ny, S.nz, 4096, 4096, P.bx, P.by, e.spread_block),
+      MM_REQ(choose_spread_blocks(S.nx, S.ny, S.nz, 4096, 4096, P.bx, P.by, e.spread_block,
+                                  e.spread_halo ? S.order : 0),
              "mesh dimensions give no valid spreading column blocks");
       P.nbx = S.nx / P.bx;
       P.nby = S.ny / P.by;
       P.bin0 = bin0;
       bin0 += P.nbx * P.nby;
+      P.halo0 = halo0;
+      if (e.spread_halo)
+        halo0 += (long)P.nbx * P.nby * (P.bx + S.order - 1) * (P.by + S.order - 1) * S.nz;
     } else {
       MM_REQ(S.kmode == 2, "kmode must be 0 (Ewald), 1 (P3M) or 2 (none)");
     }
@@ -589,6 +593,7 @@ int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h) 
   e.cmesh_total = cmesh0;
   e.green_total = green0;
   e.nbin_total = bin0;
+  e.halo_total = halo0;
   e.nbcap = (maxnb + 15) / 16 * 16;
   // partial-sum slots: finish blocks and x-pass blocks
   int nb = (e.maxg + 63) / 64;
@@ -624,7 +629,7 @@ int engine_check_layout(const Engine &e, const HostLayout &h) {
     }
     return 0;
   };
-  std::vector<Slice> site, group, cell, st, tb, lj, kv, mesh, cmesh, gr, tw, bin;
+  std::vector<Slice> site, group, cell, st, tb, lj, kv, mesh, cmesh, gr, tw, bin, halo;
   for (int s = 0; s < e.nsys; s++) {
     const SysParams &P = e.hsp[s];
     site.push_back({P.site0, P.nsite, s});
@@ -641,7 +646,10 @@ int engine_check_layout(const Engine &e, const HostLayout &h) {
       gr.push_back({P.green0, (long)P.nx * P.ny * nzh, s});
       tw.push_back({P.twid0, 3L * 128, s});
       bin.push_back({P.bin0, (long)P.nbx * P.nby, s});
-      if (P.bx * P.nbx != P.nx || P.by * P.nby != P.ny || P.bx * P.by * P.nz > 4096) {
+      const long hpts = (long)(P.bx + P.order - 1) * (P.by + P.order - 1) * P.nz;
+      if (e.spread_halo) halo.push_back({P.halo0, (long)P.nbx * P.nby * hpts, s});
+      if (P.bx * P.nbx != P.nx || P.by * P.nby != P.ny || P.bx * P.by * P.nz > 4096 ||
+          (e.spread_halo && hpts > SPREAD_MAXHALO)) {
         set_last_error("layout: spreading blocks of system " + std::to_string(s) +
                        " do not tile its mesh");
         return -1;
@@ -660,6 +668,7 @@ int engine_check_layout(const Engine &e, const HostLayout &h) {
   MM_TRY(check("influence function", gr, (long)h.green.size()));
   MM_TRY(check("twiddles", tw, (long)h.twid.size()));
   MM_TRY(check("spreading bins", bin, e.nbin_total));
+  MM_TRY(check("spreading halos", halo, e.halo_total));
   MM_REQ((long)h.kvec.size() == 4L * e.nk_total, "layout: wave vector count");
   MM_REQ((long)h.gfirst.size() == e.ngroup && (long)h.q.size() == e.nsite, "layout: site tables");
   return 0;
@@ -750,6 +759,7 @@ int engine_create(Engine &e, const std::vector<SystemInput> &in) {
   MM_TRY(dalloc(&e.d_sgrid, N));
   MM_TRY(dalloc(&e.d_sfrac, N));
   MM_TRY(dalloc(&e.d_bin_fill, e.nbin_total + 1));
+  MM_TRY(dalloc(&e.d_halo, (size_t)e.halo_total));
   e.bcap = 32;
   for (auto &P : e.hsp)
     if (P.kmode == 1) e.bcap = std::max(e.bcap, p3m_bin_capacity(P.nsite, P.nbx * P.nby));
@@ -960,7 +970,7 @@ void engine_destroy(Engine &e) {
                   e.d_nnb, e.d_rebuild,
                   e.d_err, e.d_gpos, e.d_spk, e.d_gmeta, e.d_pf, e.d_plj, e.d_gacc, e.d_tab,
                   e.d_ljt, e.d_kres, e.d_kvec, e.d_ksc, e.d_mesh, e.d_cmesh, e.d_green, e.d_twid,
-                  e.d_sgrid, e.d_sfrac, e.d_bin_fill,
+                  e.d_sgrid, e.d_sfrac, e.d_bin_fill, e.d_halo,
                   e.d_bgrid, e.d_bfrac, e.d_acc, e.d_part, e.d_fpart, e.d_done, e.d_phi, e.d_scratch,
                   e.d_nlb, e.d_ladder, e.d_kT, e.d_tail, e.d_rtmp, e.d_mom, e.d_perm,
                   e.d_remd_stats, e.d_tile_start, e.d_snap, e.d_snap_perm, e.d_snap_stats,

@@ -115,6 +115,8 @@ struct Engine {
   int4 *d_sgrid = nullptr;    // per site: jx jy jz bin
   float4 *d_sfrac = nullptr;  // per site: sx sy sz q
   int nbin_total = 0;
+  long halo_total = 0;
+  unsigned long long *d_halo = nullptr;   // spreading blocks' halo partials (SysParams::halo0)
   int *d_bin_fill = nullptr;   // sites per P3M column bin (cleared by the gather)
   int4 *d_bgrid = nullptr;    // bin-sorted copies of d_sgrid (w = site index) ...
   float4 *d_bfrac = nullptr;  // ... and of d_sfrac
@@ -132,8 +134,9 @@ struct Engine {
   int pair_blocks = 0;        // persistent grid (a multiple of 8)
   // shape (mm_shape): mean groups per neighbour-list cell (2: the
   // reference's cell count), pair workgroups per CU, spreading column-block
-  // side in mesh columns, MD steps per captured graph
-  int cell_groups = 2, pair_slots = 3, spread_block = 2, graph_steps = 10;
+  // side in mesh columns, MD steps per captured graph, P3M spreading by
+  // owner into halos (1) or by pulling neighbouring bins' sites (0)
+  int cell_groups = 2, pair_slots = 3, spread_block = 2, graph_steps = 10, spread_halo = 0;
   int *d_tile_start = nullptr;   // nsys + 1
   int nblk_finish = 0;         // finish blocks per system (max)
   int nblk_part = 0;           // partial-sum slots per system

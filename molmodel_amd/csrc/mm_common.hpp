@@ -60,6 +60,8 @@ struct SysParams {
   double T[9];                 // diag(n) L^-1, tensor_t layout (pppm.c:523-525)
   double RL[9];                // reciprocal lattice 2 pi (L^-1)^T, tensor_t layout
   int bx, by, nbx, nby, bin0;  // spread column blocks (bx x by mesh columns each)
+  long halo0;                  // the blocks' fixed-point halo partials: block k at
+                               // halo0 + k (bx+order-1)(by+order-1) nz
   int twid0;                   // first of this system's 3 x 128 FFT twiddles
   int water;                   // every group is a three-site water (mask bytes WATER_MASKS)
   int ik;                      // P3M field by -i k on the mesh: four real meshes (phi, Ex, Ey,
@@ -78,12 +80,18 @@ __host__ __device__ inline int mod_nonneg(int n, int m) {   // fns.h:48-58
   return k > 0 ? m - k : 0;
 }
 
+// A spreading block accumulates the sites of its bx x by mesh columns into
+// the (bx + order - 1) x (by + order - 1) columns their stencils reach, all
+// nz points of each, as 64-bit fixed point in LDS: at most this many points.
+constexpr int SPREAD_MAXHALO = 8192;
+
 // Mesh columns per spreading block along x and y: the smallest factor >= want
 // (at most 16) of each mesh size, else the smallest factor >= 2, falling back
 // to single columns where the block's points (bx by nz, 64-bit each in LDS)
-// would exceed maxpts or the block count would exceed maxbins.
+// would exceed maxpts, its halo (for an order > 0) SPREAD_MAXHALO, or the
+// block count maxbins.
 inline bool choose_spread_blocks(int nx, int ny, int nz, int maxpts, int maxbins, int &bx,
-                                 int &by, int want = 2) {
+                                 int &by, int want = 2, int order = 0) {
   auto factor = [want](int n) {
     for (int b = want; b <= 16; b++)
       if (n % b == 0) return b;
@@ -91,11 +99,15 @@ inline bool choose_spread_blocks(int nx, int ny, int nz, int maxpts, int maxbins
       if (n % b == 0) return b;
     return 1;
   };
+  const int h = order > 0 ? order - 1 : 0;
+  auto fits = [&]() {
+    return bx * by * nz <= maxpts && (order <= 0 || (bx + h) * (by + h) * nz <= SPREAD_MAXHALO);
+  };
   bx = factor(nx);
   by = factor(ny);
-  if (bx * by * nz > maxpts) bx = 1;
-  if (bx * by * nz > maxpts) by = 1;
-  return bx * by * nz <= maxpts && (nx / bx) * (ny / by) <= maxbins;
+  if (!fits()) bx = 1;
+  if (!fits()) by = 1;
+  return fits() && (nx / bx) * (ny / by) <= maxbins;
 }
 
 // slots per P3M column bin: four times the mean occupancy plus a margin,

@@ -6,11 +6,18 @@
 //            each site is counted into the mesh column block that holds its
 //            base point;
 //   scan / scatter   counting sort of sites into column-block bins;
-//   spread   one workgroup per column block (bx x by x nz points) pulls the
-//            sites of its own and neighbouring bins and accumulates
-//            q w_x w_y w_z into the points it owns in LDS as 64-bit fixed
-//            point (exact, so the mesh is independent of visiting order),
-//            then writes them once; no global atomics;
+//   spread   one workgroup per column block (bx x by columns), as 64-bit
+//            fixed point in LDS (exact, so independent of the order of the
+//            atomics), no global atomics, one of two ways (a shape choice):
+//            pulling -- the block takes the sites of its own and the
+//            neighbouring bins and accumulates q w_x w_y w_z into the
+//            points it owns, then writes them once; or by owner -- the
+//            block takes only its own bin's sites, accumulates their whole
+//            stencils into a halo of (bx + P - 1) x (by + P - 1) columns
+//            and writes the halo to its own slice, which the z pass sums;
+//            pulling re-reads every site for each block its stencil
+//            reaches, the halos cost their traffic, so columns holding many
+//            sites favour pulling and few sites the halos;
 //   fft      Stockham mixed-radix (4,2,3,5,7) line FFTs in LDS: z (real to
 //            half spectrum), y, then x forward, multiply by the influence
 //            function (and form the mesh energy/virial), x inverse, y
@@ -251,6 +258,67 @@ __global__ void __launch_bounds__(256) k_p3m_spread(const SysParams *__restrict_
   }
 }
 
+// Spreading by owner: the block of column block k takes only the sites
+// binned to it (base point in its bx x by columns), so every site is read
+// and weighted once, and accumulates all P^3 weights of each into an LDS
+// halo of (bx + P - 1) x (by + P - 1) columns of nz points (halo column
+// (hx, hy) is mesh column (X0 + istart + hx, Y0 + istart + hy) mod n) as
+// 64-bit fixed point, exact and so independent of the order of the
+// atomics.  The halo goes out whole to the block's own slice of `halo`;
+// the z pass adds the halos that cover each mesh column (integer sums, so
+// the mesh equals that of a single accumulator).  One work item per (site,
+// z offset c): the P lanes of a site write P consecutive points of a
+// column, in different LDS banks.
+template <int P>
+__global__ void __launch_bounds__(256) k_p3m_spread_halo(const SysParams *__restrict__ sp,
+                                                         const int *__restrict__ fill, int bcap,
+                                                         const int4 *__restrict__ bgrid,
+                                                         const float4 *__restrict__ bfrac,
+                                                         unsigned long long *__restrict__ halo) {
+  extern __shared__ unsigned long long hacc[];
+  const int s = blockIdx.y;
+  const SysParams &Q = sp[s];
+  if (Q.kmode != 1 || Q.order != P) return;
+  const int blk = blockIdx.x;
+  if (blk >= Q.nbx * Q.nby) return;
+  const int nz = Q.nz, HX = Q.bx + P - 1, HY = Q.by + P - 1;
+  const int npts = HX * HY * nz;
+  for (int t = threadIdx.x; t < npts; t += blockDim.x) hacc[t] = 0ull;
+  __syncthreads();
+  const int X0 = (blk / Q.nby) * Q.bx, Y0 = (blk % Q.nby) * Q.by;
+  const int b = Q.bin0 + blk;
+  const int n = min(fill[b], bcap);
+  constexpr int istart = -((P + 1) / 2 - 1);
+  for (int u = threadIdx.x; u < n * P; u += blockDim.x) {
+    const int site = u / P, c = u - site * P;
+    const int4 g = bgrid[(long)b * bcap + site];
+    const float4 f = bfrac[(long)b * bcap + site];
+    float wx[P], wy[P], wz[P];
+    bspline_w<P>(f.x, wx, nullptr);
+    bspline_w<P>(f.y, wy, nullptr);
+    bspline_w<P>(f.z, wz, nullptr);
+    float wc = wz[0];
+#pragma unroll
+    for (int k = 1; k < P; k++)
+      if (c == k) wc = wz[k];
+    const int iz = wrap(g.z + istart + c, nz);
+    unsigned long long *base = hacc + ((long)(g.x - X0) * HY + (g.y - Y0)) * nz + iz;
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+      const float qa = f.w * wx[a];
+#pragma unroll
+      for (int bb = 0; bb < P; bb++) {
+        const float qw = qa * wy[bb];
+        atomicAdd(base + ((long)a * HY + bb) * nz,
+                  (unsigned long long)__float2ll_rn(qw * wc * FIX_SCALE));
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long *dst = halo + Q.halo0 + (long)blk * npts;
+  for (int t = threadIdx.x; t < npts; t += blockDim.x) dst[t] = hacc[t];
+}
+
 // ---------------- line FFTs in LDS ----------------
 constexpr int FFT_T = 16;                 // threads per line
 constexpr int FFT_LINES = P3M_FFT_LINES;  // lines per block
@@ -341,15 +409,73 @@ __device__ __forceinline__ void load_tw(float2 *stw, const float2 *tw, int n) {
   for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
 }
 
-// pass 0: z lines, real -> half spectrum.  pass 1/3: y lines fwd/inv.
-// pass 4: z lines, half spectrum -> real.  blockIdx.z: the mesh component
-// (phi, then Ex, Ey, Ez of a system whose field is formed as -i k phi on
-// the mesh; the forward passes run over component 0 only).
-template <bool POW2>
+// The z line `line` (mesh column xi = line / ny, yi = line % ny) of a P3M
+// system as the sum of the spreading blocks' halo columns that cover it:
+// halo column (dx, dy) of block (bi, bj) is mesh column
+// (bi bx + istart + dx, bj by + istart + dy) mod n, so the covering blocks
+// have dx = xi - istart (mod bx), dy likewise; up to 4 x 4 of them are
+// read at once (all loads in flight), more in further rounds.  Integer
+// sums, then the conversion a single accumulator would make.
+__device__ __forceinline__ void halo_line(const SysParams &P,
+                                          const unsigned long long *__restrict__ halo, int line,
+                                          int tid, float2 *xl) {
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const int xi = line / ny, yi = line % ny;
+  const int istart = -((P.order + 1) / 2 - 1);
+  const int bxs = P.bx, bys = P.by, HX = bxs + P.order - 1, HY = bys + P.order - 1;
+  const int hpts = HX * HY * nz;
+  const unsigned long long *h = halo + P.halo0;
+  constexpr int NK = FFT_MAXN / FFT_T;
+  unsigned long long acc[NK];
+#pragma unroll
+  for (int k = 0; k < NK; k++) acc[k] = 0ull;
+  const int dx0 = mod_nonneg(xi - istart, bxs), dy0 = mod_nonneg(yi - istart, bys);
+  for (int dxb = dx0; dxb < HX; dxb += 4 * bxs)
+    for (int dyb = dy0; dyb < HY; dyb += 4 * bys) {
+      int off[16];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int dx = dxb + i * bxs;
+        const int bi = mod_nonneg(xi - istart - dx, nx) / bxs;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int dy = dyb + j * bys;
+          const int bj = mod_nonneg(yi - istart - dy, ny) / bys;
+          off[4 * i + j] =
+              (dx < HX && dy < HY) ? (bi * P.nby + bj) * hpts + (dx * HY + dy) * nz : -1;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NK; k++) {
+        const int t = tid + k * FFT_T;
+        if (t < nz) {
+          unsigned long long v = 0ull;
+#pragma unroll
+          for (int m = 0; m < 16; m++)
+            if (off[m] >= 0) v += h[off[m] + t];
+          acc[k] += v;
+        }
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    const int t = tid + k * FFT_T;
+    if (t < nz) xl[t] = make_float2((float)((double)(long long)acc[k] * FIX_INV), 0.f);
+  }
+}
+
+// pass 0: z lines, real -> half spectrum; HALO: the line is the sum of the
+// spreading blocks' halo columns that cover it (halo_line), else the real
+// mesh.  pass 1/3: y lines fwd/inv.  pass 4: z lines, half
+// spectrum -> real.  blockIdx.z: the mesh component (phi, then Ex, Ey, Ez
+// of a system whose field is formed as -i k phi on the mesh; the forward
+// passes run over component 0 only).
+template <bool POW2, bool HALO = false>
 __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restrict__ sp,
                                                      const float2 *__restrict__ twid_all,
                                                      float *__restrict__ mesh,
-                                                     float2 *__restrict__ cmesh, int pass) {
+                                                     float2 *__restrict__ cmesh, int pass,
+                                                     const unsigned long long *__restrict__ halo) {
   __shared__ float2 bx_[FFT_LINES * FFT_MAXN], by_[FFT_LINES * FFT_MAXN];
   __shared__ float2 stw[FFT_MAXN];
   const int s = blockIdx.y;
@@ -372,7 +498,9 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
   const int sign = (pass == 3 || pass == 4) ? 1 : -1;
   // load
   if (pass == 0) {
-    if (on) {
+    if constexpr (HALO) {
+      if (on) halo_line(P, halo, line, tid, xl);
+    } else if (on) {
       const float *src = mesh + mesh0 + (long)line * nz;
       for (int t = tid; t < n; t += FFT_T) xl[t] = make_float2(src[t], 0.f);
     }
@@ -640,20 +768,27 @@ static void launch_xgreen(Engine &e, int want_virial, int energies, hipStream_t 
                        e.d_kres + 4L * e.nsite, want_virial, e.nblk_part);
 }
 
+// halo: the spreading blocks' partials the z pass sums (null: the real
+// mesh holds the charges)
 template <bool POW2>
 static void launch_fft(Engine &e, int want_virial, int energies, hipStream_t st, int maxx,
-                       int maxy, int maxz, const SysParams *sp, int nsg, bool ik = false) {
-  hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, nsg), dim3(FFT_TB), 0, st, sp,
-                     e.d_twid, e.d_mesh, e.d_cmesh, 0);
+                       int maxy, int maxz, const SysParams *sp, int nsg, bool ik,
+                       const unsigned long long *halo) {
+  if (halo)
+    hipLaunchKernelGGL((k_fft_pass<POW2, true>), dim3(maxz, nsg), dim3(FFT_TB), 0, st, sp,
+                       e.d_twid, e.d_mesh, e.d_cmesh, 0, halo);
+  else
+    hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, nsg), dim3(FFT_TB), 0, st, sp,
+                       e.d_twid, e.d_mesh, e.d_cmesh, 0, nullptr);
   hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, nsg), dim3(FFT_TB), 0, st, sp,
-                     e.d_twid, e.d_mesh, e.d_cmesh, 1);
+                     e.d_twid, e.d_mesh, e.d_cmesh, 1, nullptr);
   if (ik) launch_xgreen<POW2, true>(e, want_virial, energies, st, dim3(maxx, nsg), sp);
   else launch_xgreen<POW2, false>(e, want_virial, energies, st, dim3(maxx, nsg), sp);
   const int ncomp = ik ? 4 : 1;
   hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxy, nsg, ncomp), dim3(FFT_TB), 0, st, sp,
-                     e.d_twid, e.d_mesh, e.d_cmesh, 3);
+                     e.d_twid, e.d_mesh, e.d_cmesh, 3, nullptr);
   hipLaunchKernelGGL((k_fft_pass<POW2>), dim3(maxz, nsg, ncomp), dim3(FFT_TB), 0, st, sp,
-                     e.d_twid, e.d_mesh, e.d_cmesh, 4);
+                     e.d_twid, e.d_mesh, e.d_cmesh, 4, nullptr);
 }
 
 int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies) {
@@ -682,20 +817,32 @@ int launch_p3m(Engine &e, int want_virial, hipStream_t st, int energies) {
       (p.ik ? used_ik : used_ad)[p.order] = true;
       ik |= p.ik != 0;
     }
-#define SPREAD(PO)                                                                          \
-  if (used[PO])                                                                             \
-    hipLaunchKernelGGL(k_p3m_spread<PO>, dim3(maxbins, e.nsys), dim3(256), 0, st,     \
-                       e.d_sp, e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_mesh);
+  // LDS of the spreading: the largest halo of the systems of each order
+  size_t hbytes[MAXORDER + 1] = {0};
+  for (auto &p : e.hsp)
+    if (p.kmode == 1)
+      hbytes[p.order] = std::max(hbytes[p.order], sizeof(unsigned long long) *
+                                                      (p.bx + p.order - 1) * (p.by + p.order - 1) * p.nz);
+#define SPREAD(PO)                                                                           \
+  if (used[PO] && e.spread_halo)                                                             \
+    hipLaunchKernelGGL(k_p3m_spread_halo<PO>, dim3(maxbins, e.nsys), dim3(256), hbytes[PO],   \
+                       st, e.d_sp, e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_halo);     \
+  else if (used[PO])                                                                         \
+    hipLaunchKernelGGL(k_p3m_spread<PO>, dim3(maxbins, e.nsys), dim3(256), 0, st, e.d_sp,    \
+                       e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_mesh);
   SPREAD(1) SPREAD(2) SPREAD(3) SPREAD(4) SPREAD(5) SPREAD(6) SPREAD(7)
 #undef SPREAD
+  const unsigned long long *halo = e.spread_halo ? e.d_halo : nullptr;
   // meshes whose sides are all powers of two take FFT kernels built with
   // the radix-4/2 stages only (fewer registers)
   bool pow2 = true;
   for (auto &p : e.hsp)
     if (p.kmode == 1)
       for (int n : {p.nx, p.ny, p.nz}) pow2 &= (n & (n - 1)) == 0;
-  if (pow2) launch_fft<true>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys, ik);
-  else launch_fft<false>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys, ik);
+  if (pow2)
+    launch_fft<true>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys, ik, halo);
+  else
+    launch_fft<false>(e, want_virial, energies, st, maxx, maxy, maxz, e.d_sp, e.nsys, ik, halo);
 #define GATHER(PO)                                                                        \
   if (used_ad[PO])                                                                        \
     hipLaunchKernelGGL(k_p3m_gather<PO>, gs, dim3(tb), 0, st, e.d_sp, e.d_sgrid,     \
@@ -745,8 +892,8 @@ int launch_p3m_adjoint(Engine &e, int sys, const float4 *src, hipStream_t st) {
   for (auto &p : e.hsp)
     if (p.kmode == 1)
       for (int n : {p.nx, p.ny, p.nz}) pow2 &= (n & (n - 1)) == 0;
-  if (pow2) launch_fft<true>(e, 0, 0, st, maxx, maxy, maxz, sp, 1);
-  else launch_fft<false>(e, 0, 0, st, maxx, maxy, maxz, sp, 1);
+  if (pow2) launch_fft<true>(e, 0, 0, st, maxx, maxy, maxz, sp, 1, false, nullptr);
+  else launch_fft<false>(e, 0, 0, st, maxx, maxy, maxz, sp, 1, false, nullptr);
 #define ADJ_GATHER(PO)                                                                      \
   case PO:                                                                                  \
     hipLaunchKernelGGL(k_p3m_gather<PO>, gs, dim3(tb), 0, st, sp, e.d_sgrid, e.d_sfrac,     \

@@ -4,7 +4,8 @@
 // cell size (mean groups per cell), the pair kernel's persistent grid
 // (workgroups per CU, which also sets how many slots the reciprocal-space
 // kernels keep), the P3M spreading column blocks (mesh columns per block
-// side) and the MD steps captured per graph.  None of them changes what is
+// side) and their way of spreading (by owner into halos, or by pulling the
+// sites whose stencils reach a block), and the MD steps captured per graph.  None of them changes what is
 // computed: pair-kernel tiles and spreading blocks give bit-identical
 // forces, graph blocks bit-identical trajectories, and other cell sizes
 // give the same pair set up to pairs whose FP64 distance lies within
@@ -74,7 +75,8 @@ extern "C" int mm_tune_shape(int nsys, const mm_system_desc *sys, const double *
   mm_shape cur;
   mm_shape_default(&cur);
   auto key = [](const mm_shape &s) {
-    return std::make_tuple(s.cell_groups, s.pair_slots, s.spread_block, s.graph_steps);
+    return std::make_tuple(s.cell_groups, s.pair_slots, s.spread_block, s.graph_steps,
+                           s.spread_halo);
   };
   std::map<decltype(key(cur)), double> done;
   int nt = 0;
@@ -96,16 +98,18 @@ extern "C" int mm_tune_shape(int nsys, const mm_system_desc *sys, const double *
   };
   double tbest = measure(cur);
   // one choice at a time, in the order of their measured effect
-  enum { SLOTS, SPREAD, CELLS, GRAPH };
+  enum { SLOTS, SPREAD, HALO, CELLS, GRAPH };
   const std::vector<std::pair<int, std::vector<int>>> knobs = {
-      {SLOTS, {2, 3, 4}}, {SPREAD, {2, 4}}, {CELLS, {1, 2, 4}}, {GRAPH, {5, 10, 20}}};
+      {SLOTS, {2, 3, 4}}, {SPREAD, {2, 4}}, {HALO, {0, 1}}, {CELLS, {1, 2, 4}},
+      {GRAPH, {5, 10, 20}}};
   for (const auto &kv : knobs) {
-    if (kv.first == SPREAD && !any_p3m) continue;
+    if ((kv.first == SPREAD || kv.first == HALO) && !any_p3m) continue;
     mm_shape keep = cur;
     for (int val : kv.second) {
       mm_shape s = cur;
       if (kv.first == SLOTS) s.pair_slots = val;
       if (kv.first == SPREAD) s.spread_block = val;
+      if (kv.first == HALO) s.spread_halo = val;
       if (kv.first == CELLS) s.cell_groups = val;
       if (kv.first == GRAPH) s.graph_steps = val;
       const double t = measure(s);

@@ -106,7 +106,7 @@ def _ptr(a):
 
 class Shape(C.Structure):
     _fields_ = [("cell_groups", _I), ("pair_slots", _I), ("spread_block", _I),
-                ("graph_steps", _I)]
+                ("graph_steps", _I), ("spread_halo", _I)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -49,3 +49,22 @@ def test_batch_equals_single_system_engines(gpu_lib):
               "max diff", np.abs(fb[off[k]:off[k + 1]] - f1).max(), "energies", eb[k][:4], e1[:4])
         assert same_f
         assert np.array_equal(eb[k][:4], e1[:4])
+
+
+@pytest.mark.parametrize("shape", [dict(spread_halo=1), dict(spread_halo=1, spread_block=4)])
+def test_spreading_by_owner_equals_pulling(gpu_lib, shape):
+    """Spreading by owner into halos (summed by the z pass) gives the mesh
+    of spreading by pulling bit for bit, for every system of the mixed
+    batch (orders 5 and 7, meshes 24^3 and 30^3, Ewald systems between)."""
+    mem = members()
+    systems = [m[0] for m in mem]
+    params = [m[1] for m in mem]
+    r = np.concatenate([s.r for s in systems])
+    out = []
+    for sh in (None, shape):
+        e = Engine(systems, params, shape=sh)
+        e.set_positions(r)
+        e.compute(virial=True)
+        out.append((e.forces(), e.energies(), e.virial()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])

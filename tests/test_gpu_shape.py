@@ -2,9 +2,10 @@
 
 A shape changes how the batch is laid onto the device, not what is
 computed:
-  * pair workgroups per CU and P3M spreading blocks: forces bit-identical
-    (a pair tile's sums do not depend on the workgroup that takes it; the
-    mesh is accumulated in exact fixed point);
+  * pair workgroups per CU, P3M spreading blocks and spreading by owner
+    into halos instead of by pulling: forces bit-identical (a pair tile's
+    sums do not depend on the workgroup that takes it; the mesh is
+    accumulated in exact fixed point, the halos summed as integers);
   * steps per captured graph: trajectories bit-identical;
   * neighbour-list cells of 1 or 4 groups on average instead of the
     reference's 2: the same unordered pair set as the oracle's build (which
@@ -46,7 +47,9 @@ def p3m_box():
 
 
 @pytest.mark.parametrize("shape", [dict(pair_slots=2), dict(pair_slots=4), dict(pair_slots=1),
-                                   dict(spread_block=4), dict(spread_block=1)])
+                                   dict(spread_block=4), dict(spread_block=1),
+                                   dict(spread_halo=1), dict(spread_halo=1, spread_block=4),
+                                   dict(spread_halo=1, spread_block=1)])
 def test_tile_and_spread_shapes_are_bit_identical(gpu_lib, p3m_box, shape):
     s, p, f0, en0, _ = p3m_box
     print(f"[shape] {shape}", flush=True)
@@ -93,9 +96,10 @@ def test_tuner_picks_a_timed_shape(gpu_lib, name):
     shapes = [t for t, _ in trials]
     assert best in shapes and all(ms > 0 for _, ms in trials)
     assert trials[0][0] == E.default_shape()
-    # one choice at a time: pair slots (3), spreading blocks for P3M (2),
-    # cells (3), graph steps (3), each shape timed once
+    # one choice at a time: pair slots (3), spreading blocks and the way of
+    # spreading for P3M (2 + 2), cells (3), graph steps (3), each shape
+    # timed once
     assert len(shapes) == len({tuple(t.values()) for t in shapes})
-    assert len(shapes) >= (7 if p["kmode"] == 0 else 8)
+    assert len(shapes) >= (7 if p["kmode"] == 0 else 9)
     ms = dict((tuple(t.values()), m) for t, m in trials)
     assert ms[tuple(best.values())] == min(ms.values())

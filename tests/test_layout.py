@@ -67,13 +67,14 @@ def test_invalid_input_is_reported():
 
 
 def test_default_shape_and_shape_checks():
-    assert E.default_shape() == dict(cell_groups=2, pair_slots=3, spread_block=2, graph_steps=10)
+    assert E.default_shape() == dict(cell_groups=2, pair_slots=3, spread_block=2, graph_steps=10,
+                                     spread_halo=0)
     s, p = W.config("water648_ewald")
     d, keep = E.system_desc(s, p)
     h = C.c_void_p()
     L = E.lib()
     for bad in (dict(pair_slots=0), dict(pair_slots=5), dict(cell_groups=0),
-                dict(spread_block=9), dict(graph_steps=65)):
+                dict(spread_block=9), dict(graph_steps=65), dict(spread_halo=2)):
         sh = E._shape(bad)
         rc = L.mm_engine_create_shaped(1, C.byref(d), C.byref(sh), C.byref(h))
         assert rc == -1 and "mm_shape" in L.mm_last_error().decode()
