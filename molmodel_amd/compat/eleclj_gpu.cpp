@@ -95,7 +95,6 @@ void ElecLJGpu::init(MSys *m)
     fatal("1-4 scale factors other than 1 are not supported");
   if (!par.include_rspace || !par.include_kspace || !par.include_self)
     fatal("the real-space, k-space and self terms are always evaluated");
-  if (par.use_p3m && par.p3m_ik_differentiate) fatal("ik differentiation is not supported");
 
   /* eleclj.cpp:643-771: sites, groups, 1-2 and 1-3 exclusions */
   const int natom = m->atom.size();
@@ -141,6 +140,7 @@ void ElecLJGpu::init(MSys *m)
   d.kc = par.kspace_cutoff;
   d.order = par.p3m_assignment_order;
   d.nalias = par.p3m_alias_sum_limit;
+  d.ik = par.p3m_ik_differentiate ? 1 : 0;
   d.include_lj_correction = par.include_lj_correction ? 1 : 0;
   d.geometric_sigma = par.geometric_combining_for_sigma ? 1 : 0;
   check(mm_desc_resolve(&d, par.p3m_grid_spacing, par.p3m_grid_density), "mm_desc_resolve");

@@ -11,12 +11,12 @@
  *
  * Covered: cubic periodic boxes; atom sites with fixed charges and LJ
  * sigma/epsilon by type (general_type aliases included); Ewald or P3M with
- * real-space differentiation; the LJ tail correction; box scaling; types
+ * real-space or ik differentiation; the LJ tail correction; box scaling; types
  * and lambda changes (types_changed(perturbed) + set_lambda).  Anything
  * else that ElecLJ accepts -- virtual (M, sp2, sp3) sites, bond charge
  * increments, 1-3 interactions, screening radii, 1-4 scaling, non-periodic
- * or non-cubic boxes, ik differentiation, switching off the real-space,
- * k-space or self terms -- stops the program with a message in init().
+ * or non-cubic boxes, switching off the real-space, k-space or self
+ * terms -- stops the program with a message in init().
  *
  * Built inside the reference's tree with its headers on the include path
  * (eleclj.hpp, msys.hpp, ...) and linked with libmolmodel_amd.so.
