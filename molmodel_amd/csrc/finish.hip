@@ -64,12 +64,23 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
     const int s0 = gfirst[g], ns = gns[g];
     constexpr int K = 3;
     double F[K][3], ph[K], qq[K], x[K][3];
+    // every load is issued here, before any arithmetic: one wave per SIMD
+    // leaves no other wave to hide a late load's latency
+    double vv[K][3], m[K];
+    const int gin = ginfo[g];
+    const double l0 = last[3 * g], l1 = last[3 * g + 1], l2 = last[3 * g + 2];
 #pragma unroll
     for (int k = 0; k < K; k++) {
       F[k][0] = F[k][1] = F[k][2] = ph[k] = qq[k] = 0;
       x[k][0] = x[k][1] = x[k][2] = 0;
+      vv[k][0] = vv[k][1] = vv[k][2] = 0;
+      m[k] = 1;
       if (k < ns) {
         const int sk = s0 + k;
+        if (MODE) {
+          vv[k][0] = v[sk]; vv[k][1] = v[N + sk]; vv[k][2] = v[2 * N + sk];
+          m[k] = mass[sk];
+        }
         const double *o = pf + ((long)gs * MAXS + k) * 4;
         const double *kr = kres + 4L * sk;
         qq[k] = q[sk];
@@ -134,17 +145,6 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
       }
     if (MODE) {
       const double dt = P.dt;
-      double vv[K][3], m[K];
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        vv[k][0] = vv[k][1] = vv[k][2] = 0;
-        m[k] = 1;
-        if (k < ns) {
-          const int sk = s0 + k;
-          vv[k][0] = v[sk]; vv[k][1] = v[N + sk]; vv[k][2] = v[2 * N + sk];
-          m[k] = mass[sk];
-        }
-      }
       const bool water = (ns == 3);
       if (MODE & FIN_KICK) {
 #pragma unroll
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
           if (k < K && k < ns)
             o = make_float4((float)(x[k][0] - x[0][0]), (float)(x[k][1] - x[0][1]),
                             (float)(x[k][2] - x[0][2]),
-                            ((ginfo[g] >> (8 * k)) & SITE_HAS_Q) ? (float)qq[k] : 0.f);
+                            ((gin >> (8 * k)) & SITE_HAS_Q) ? (float)qq[k] : 0.f);
           spk[(long)gs * MAXS + k] = o;
         }
       }
@@ -195,8 +195,7 @@ __global__ void __launch_bounds__(FIN_TB) k_finish(
         }
     }
     // nbrlist.cpp:64-73 on the first site
-    const double dx = x[0][0] - last[3 * g], dy = x[0][1] - last[3 * g + 1],
-                 dz = x[0][2] - last[3 * g + 2];
+    const double dx = x[0][0] - l0, dy = x[0][1] - l1, dz = x[0][2] - l2;
     vals[TOP] = sqrt(dx * dx + dy * dy + dz * dz);
   }
   if constexpr (LAZY) {
