@@ -24,9 +24,11 @@ def scaled_positions(s, f):
     return r
 
 
-@pytest.mark.parametrize("name", ["water648_ewald", "water12k_p3m"])
-def test_scaled_box_matches_fresh_engine_and_oracle(gpu_lib, name):
+@pytest.mark.parametrize("name,ik", [("water648_ewald", 0), ("water12k_p3m", 0),
+                                     ("water12k_p3m", 1)])
+def test_scaled_box_matches_fresh_engine_and_oracle(gpu_lib, name, ik):
     s, p = W.config(name)
+    p = dict(p, ik=ik)
     f = 1.015
     e = Engine(s, p)
     e.set_positions(s.r)
