@@ -234,10 +234,9 @@ int mm_engine_create(int nsys, const mm_system_desc *sys, mm_engine **out);
  *                 sums; pays where columns hold few sites) or by pulling
  *                 (0: each block gathers the sites whose stencils reach its
  *                 columns); mesh bit-identical.
- * mm_shape_default: 2, 3, 2, 10, 0 (MOLMODEL_AMD_PAIR_SLOTS,
- * MOLMODEL_AMD_SPREAD_B, MOLMODEL_AMD_GRAPH_STEPS and
- * MOLMODEL_AMD_SPREAD_HALO override the last four); mm_engine_create uses
- * it. */
+ * mm_shape_default: 2, 3, 2, 10, 0 (MOLMODEL_AMD_CELL_GROUPS,
+ * MOLMODEL_AMD_PAIR_SLOTS, MOLMODEL_AMD_SPREAD_B, MOLMODEL_AMD_GRAPH_STEPS
+ * and MOLMODEL_AMD_SPREAD_HALO override them); mm_engine_create uses it. */
 typedef struct {
   int cell_groups, pair_slots, spread_block, graph_steps, spread_halo;
 } mm_shape;

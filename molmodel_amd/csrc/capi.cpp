@@ -88,6 +88,7 @@ void mm_shape_default(mm_shape *s) {
   s->spread_block = 2;
   s->graph_steps = 10;
   s->spread_halo = 0;
+  if (const char *v = getenv("MOLMODEL_AMD_CELL_GROUPS")) s->cell_groups = atoi(v);
   if (const char *v = getenv("MOLMODEL_AMD_PAIR_SLOTS")) s->pair_slots = atoi(v);
   if (const char *v = getenv("MOLMODEL_AMD_SPREAD_HALO")) s->spread_halo = atoi(v);
   if (const char *v = getenv("MOLMODEL_AMD_SPREAD_B")) s->spread_block = atoi(v);
