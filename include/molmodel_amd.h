@@ -248,8 +248,8 @@ int mm_engine_create_shaped(int nsys, const mm_system_desc *sys, const mm_shape 
  * caller's batch (positions r, velocities v or NULL), runs MD with
  * mm_md_setup(dt, bond, angle_deg, tol), and times `steps`-step blocks
  * (graph replay; the best of two after a warm-up); one choice at a time
- * from the default shape: pair_slots {2, 3, 4}, spread_block {2, 4} and
- * spread_halo {0, 1} (when a system uses P3M), cell_groups {1, 2, 4},
+ * from the default shape: pair_slots {2, 3, 4}, spread_halo {0, 1} and
+ * spread_block {2, 4, 8} (when a system uses P3M), cell_groups {1, 2, 4},
  * graph_steps {5, 10, 20}.  *best:
  * the fastest; trials[0..max_trials) the shapes timed and their ms per
  * step (-1 for a shape that could not run), *ntrials their number.  The

@@ -100,7 +100,7 @@ extern "C" int mm_tune_shape(int nsys, const mm_system_desc *sys, const double *
   // one choice at a time, in the order of their measured effect
   enum { SLOTS, SPREAD, HALO, CELLS, GRAPH };
   const std::vector<std::pair<int, std::vector<int>>> knobs = {
-      {SLOTS, {2, 3, 4}}, {SPREAD, {2, 4}}, {HALO, {0, 1}}, {CELLS, {1, 2, 4}},
+      {SLOTS, {2, 3, 4}}, {HALO, {0, 1}}, {SPREAD, {2, 4, 8}}, {CELLS, {1, 2, 4}},
       {GRAPH, {5, 10, 20}}};
   for (const auto &kv : knobs) {
     if ((kv.first == SPREAD || kv.first == HALO) && !any_p3m) continue;

@@ -97,9 +97,9 @@ def test_tuner_picks_a_timed_shape(gpu_lib, name):
     assert best in shapes and all(ms > 0 for _, ms in trials)
     assert trials[0][0] == E.default_shape()
     # one choice at a time: pair slots (3), spreading blocks and the way of
-    # spreading for P3M (2 + 2), cells (3), graph steps (3), each shape
+    # spreading for P3M (3 + 2), cells (3), graph steps (3), each shape
     # timed once
     assert len(shapes) == len({tuple(t.values()) for t in shapes})
-    assert len(shapes) >= (7 if p["kmode"] == 0 else 9)
+    assert len(shapes) >= (7 if p["kmode"] == 0 else 10)
     ms = dict((tuple(t.values()), m) for t, m in trials)
     assert ms[tuple(best.values())] == min(ms.values())
