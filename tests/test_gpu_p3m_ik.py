@@ -256,3 +256,40 @@ def test_reference_example_through_the_dropins(gpu_lib):
         print(f"[ik]   site {i}: q {q[i]: .6f} phi {phi[i]: .6f} E {ev[i, 0]: .6f} {ev[i, 1]: .6f} {ev[i, 2]: .6f}")
     assert rel(phi, ophi) < 1e-5 and rel(ev, oev) < 1e-5
     assert dF < 3 * predicted
+
+
+@pytest.mark.parametrize("mesh,order", [(32, 3), (24, 4), (20, 6), (32, 7)])
+def test_ik_orders_and_meshes(gpu_lib, mesh, order):
+    """ik forces, energy and virial against the oracle across assignment
+    orders and radix-2 / mixed-radix meshes (the x pass's four builds)."""
+    s, p = small_box(1)
+    p = dict(p, mesh=(mesh,) * 3, order=order)
+    print(f"[ik] mesh {mesh} order {order}", flush=True)
+    e = Engine(s, p)
+    e.set_positions(s.r)
+    e.compute(virial=True)
+    f, en, w = e.forces(), e.energies()[0], e.virial()[0]
+    of, oen, ow = O.evaluate(s, p, virial=True)
+    print(f"[ik] mesh {mesh} order {order}: force rel {rel(f, of):.3e} "
+          f"virial rel {rel(w, ow.reshape(3, 3)):.3e}")
+    assert rel(f, of) <= 1e-5
+    assert abs(en[0] - oen[0]) <= 1e-5 * abs(oen[3])
+    assert rel(w, ow.reshape(3, 3)) <= 1e-5
+
+
+def test_ik_spreading_forms_agree(gpu_lib):
+    """Spreading by owner into halos gives the ik systems of a mixed batch
+    the forces of spreading by pulling, bit for bit."""
+    mem = [small_box(0), small_box(1, seed=32)]
+    systems = [m[0] for m in mem]
+    params = [m[1] for m in mem]
+    r = np.concatenate([s.r for s in systems])
+    out = []
+    for sh in (None, dict(spread_halo=1), dict(spread_halo=1, spread_block=4)):
+        e = Engine(systems, params, shape=sh)
+        e.set_positions(r)
+        e.compute(virial=True)
+        out.append((e.forces(), e.energies(), e.virial()))
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0])
+        assert np.array_equal(o[1], out[0][1]) and np.array_equal(o[2], out[0][2])
