@@ -1,11 +1,11 @@
 // pppm.hip -- P3M on the device (pppm.c:498-641).
 //
 // Steps, each its own kernel on one stream:
-//   grid     s = diag(n) L^-1 r in FP64, nearest/lower mesh point and the
-//            fractional offset as the reference forms them (pppm.c:523-548);
-//            each site is counted into the mesh column block that holds its
-//            base point;
-//   scan / scatter   counting sort of sites into column-block bins;
+//   bin      s = diag(n) L^-1 r in FP64, nearest/lower mesh point and the
+//            fractional offset as the reference forms them (pppm.c:523-548),
+//            and a copy of each site in the slots of the mesh column-block
+//            bin that holds its base point (a per-block LDS rank per bin and
+//            one global claim of slots per bin the block touches);
 //   spread   one workgroup per column block (bx x by columns), as 64-bit
 //            fixed point in LDS (exact, so independent of the order of the
 //            atomics), no global atomics, one of two ways (a shape choice):
@@ -24,7 +24,9 @@
 //            inverse, z (half spectrum to real), unnormalised like FFTW;
 //   gather   phi_i = sum w phi_m and, with real-space differentiation (the
 //            reference default), E_i = (diag(n) L^-1)^T (sum -dw phi_m)
-//            (pppm.c:603-640).
+//            (pppm.c:603-640); with the field formed on the mesh (ik), the
+//            x pass also forms E_a = -i k_a phi_k, the inverse passes run
+//            over four components and E_i = sum w E_m (pppm.c:577-626).
 // Assignment weights are the cardinal B-splines that the reference's
 // polynomial tables tabulate (pppm.c:69-275), evaluated by recursion in FP32.
 #include "engine.hpp"
