@@ -184,7 +184,8 @@ def test_ik_p3m_dropin(gpu_lib, box12k):
 def test_ik_p3m_optimize_uses_the_ik_estimate(gpu_lib):
     """p3m_optimize on an ik handle minimises chi_r^2 + chi_ik^2: its
     estimates are those of the oracle's ik influence function at the
-    returned eta, and the optimum differs from the real-space
+    returned eta, the returned eta is within 1e-3 of the minimum of a fine
+    scan of that function, and the optimum differs from the real-space
     differentiated one."""
     L = lib()
     s, _ = W.config("water648_ewald")
@@ -200,6 +201,21 @@ def test_ik_p3m_optimize_uses_the_ik_estimate(gpu_lib):
         _, oc = O.green((mesh,) * 3, s.boxl, eta[0], 5, ik=ik)
         assert abs(ck[0] - oc) <= 1e-9 * oc
         out[ik] = eta[0]
+        if ik:
+            V = s.boxl ** 3
+
+            def f(x):
+                chi_r = 2 / np.sqrt(rc / V ** (1 / 3)) * np.exp(-(rc * x) ** 2)
+                return chi_r ** 2 + O.green((mesh,) * 3, s.boxl, x, 5, ik=1)[1] ** 2
+
+            xs = np.linspace(0.9 * eta[0], 1.1 * eta[0], 41)
+            vals = [f(x) for x in xs]
+            k = int(np.argmin(vals))
+            assert 0 < k < len(xs) - 1, "the scan brackets the minimum"
+            fine = np.linspace(xs[k - 1], xs[k + 1], 41)
+            x_min = fine[int(np.argmin([f(x) for x in fine]))]
+            print(f"[ik] p3m_optimize eta {eta[0]:.6f}, scan minimum {x_min:.6f}")
+            assert abs(eta[0] - x_min) <= 1e-3 * x_min
     print(f"[ik] optimal eta: real-space differentiation {out[0]:.5f}, ik {out[1]:.5f}")
     assert abs(out[0] - out[1]) > 1e-3 * out[0]
 
