@@ -227,10 +227,10 @@ def test_kspace_phi_dropin(gpu_lib):
     assert rel(phi, ophi) < 1e-11
 
 
-def p3m_handle(L, s, mesh, eta, order=5):
+def p3m_handle(L, s, mesh, eta, order=5, ik=0):
     latv = np.array([s.boxl, 0, 0, 0, s.boxl, 0, 0, 0, s.boxl])
     ng = np.zeros(3, dtype=np.int32)
-    pp = L.p3m_new(s.nsite, latv.ctypes.data_as(P), s.boxl / mesh, 2, 0, order,
+    pp = L.p3m_new(s.nsite, latv.ctypes.data_as(P), s.boxl / mesh, 2, ik, order,
                    ng.ctypes.data_as(P))
     assert list(ng) == [mesh] * 3
     chi = np.zeros(1)
@@ -260,7 +260,8 @@ def test_p3m_field_phi_only(gpu_lib):
     assert rel(phi, ophi) < 1e-5 and rel(ev2, oev) < 1e-5
 
 
-def test_p3m_scale_and_copy(gpu_lib):
+@pytest.mark.parametrize("ik", [0, 1])
+def test_p3m_scale_and_copy(gpu_lib, ik):
     """p3m_scale (pppm.c:643-664): the box times f and G / f is the P3M of
     the scaled box with eta / f (G is homogeneous in k and eta), so the field
     at scaled positions equals the oracle's for that box and eta.
@@ -271,12 +272,12 @@ def test_p3m_scale_and_copy(gpu_lib):
     L.p3m_copy.argtypes = [P, I]
     s, p = W.config("water648_ewald")
     eta, f = 0.35, 1.03
-    pp, _ = p3m_handle(L, s, 24, eta)
+    pp, _ = p3m_handle(L, s, 24, eta, ik=ik)
     L.p3m_scale(pp, f)
     t = W.System(**{k: getattr(s, k) for k in s.__dataclass_fields__})
     t.boxl, t.r = s.boxl * f, s.r * f
     phi, ev = p3m_run(L, pp, t)
-    ophi, oev, _, _ = O.p3m_parts(t, dict(mesh=(24, 24, 24), eta=eta / f, order=5))
+    ophi, oev, _, _ = O.p3m_parts(t, dict(mesh=(24, 24, 24), eta=eta / f, order=5, ik=ik))
     print("[refapi] p3m_scale rel", rel(phi, ophi), rel(ev, oev))
     assert rel(phi, ophi) < 1e-5 and rel(ev, oev) < 1e-5
     # copy to the first 27 waters, at the unscaled box again
@@ -286,7 +287,7 @@ def test_p3m_scale_and_copy(gpu_lib):
     L.p3m_delete(pp)
     phi_c, ev_c = p3m_run(L, cp, small)
     L.p3m_delete(cp)
-    sphi, sev, _, _ = O.p3m_parts(small, dict(mesh=(24, 24, 24), eta=eta, order=5))
+    sphi, sev, _, _ = O.p3m_parts(small, dict(mesh=(24, 24, 24), eta=eta, order=5, ik=ik))
     print("[refapi] p3m_copy rel", rel(phi_c, sphi), rel(ev_c, sev))
     assert rel(phi_c, sphi) < 1e-5 and rel(ev_c, sev) < 1e-5
 
