@@ -309,3 +309,21 @@ def test_ik_spreading_forms_agree(gpu_lib):
     for o in out[1:]:
         assert np.array_equal(o[0], out[0][0])
         assert np.array_equal(o[1], out[0][1]) and np.array_equal(o[2], out[0][2])
+
+
+def test_ik_with_ions(gpu_lib):
+    """1000 waters and 20 + 20 ions (single-site groups take the general
+    pair path) under P3M with the field formed on the mesh: forces,
+    energies and virial against the oracle."""
+    s, p = W.config("water1000_ions")
+    p = dict(p, kmode=1, mesh=(32, 32, 32), order=5, ik=1)
+    print(f"[ik] water1000_ions N={s.nsite}", flush=True)
+    e = Engine(s, p)
+    e.set_positions(s.r)
+    e.compute(virial=True)
+    f, en, w = e.forces(), e.energies()[0], e.virial()[0]
+    of, oen, ow = O.evaluate(s, p, virial=True)
+    print(f"[ik] ions: force rel {rel(f, of):.3e} virial rel {rel(w, ow.reshape(3, 3)):.3e}")
+    assert rel(f, of) <= 1e-5
+    assert abs(en[0] - oen[0]) <= 1e-5 * abs(oen[3])
+    assert rel(w, ow.reshape(3, 3)) <= 1e-5
