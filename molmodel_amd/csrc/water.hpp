@@ -13,6 +13,15 @@ struct WaterC {
   double g[3][9];
 };
 
+// gradient entries that can be nonzero: the O-H1 length reaches O and H1
+// (coordinates 0-5), the O-H2 length O and H2 (0-2, 6-8), the angle all
+// nine.  Products with the other entries are exact zeros, and adding a zero
+// leaves a sum unchanged, so the loops below skip them with results equal
+// to the full sums (for finite inputs).
+__host__ __device__ constexpr bool water_reach(int a, int d) {
+  return a == 2 || (a == 0 ? d < 6 : (d < 3 || d >= 6));
+}
+
 // constraint values and gradients (constraint.cpp:30-112, geograd.cpp:59-103)
 __device__ inline void water_constraints(const double *x, double bond, double t0, WaterC &c) {
   for (int a = 0; a < 3; a++)
@@ -80,8 +89,10 @@ __device__ inline bool shake_water(double *x, double *v, const double *x0, const
   WaterC c0;
   water_constraints(x0, bond, t0, c0);
   double sci0[3][9];
+#pragma unroll
   for (int a = 0; a < 3; a++)
-    for (int d = 0; d < 9; d++) sci0[a][d] = c0.g[a][d] / m[d / 3];
+#pragma unroll
+    for (int d = 0; d < 9; d++) sci0[a][d] = water_reach(a, d) ? c0.g[a][d] / m[d / 3] : 0.0;
   const double invdt = dt > 0 ? 1 / dt : 0;
   for (int it = 0; it < maxit; it++) {
     WaterC c;
@@ -91,15 +102,22 @@ __device__ inline bool shake_water(double *x, double *v, const double *x0, const
     for (int a = 0; a < 3; a++) { xs[a] = -c.val[a]; if (fabs(xs[a]) > tol) sat = false; }
     if (sat) return true;
     double s[3][3];
+#pragma unroll
     for (int a = 0; a < 3; a++)
+#pragma unroll
       for (int b = 0; b < 3; b++) {
         double t = 0;
-        for (int d = 0; d < 9; d++) t += c.g[a][d] * sci0[b][d];
+#pragma unroll
+        for (int d = 0; d < 9; d++)
+          if (water_reach(a, d) && water_reach(b, d)) t += c.g[a][d] * sci0[b][d];
         s[a][b] = t;
       }
     if (!solve3(s, xs)) return false;
+#pragma unroll
     for (int a = 0; a < 3; a++)
+#pragma unroll
       for (int d = 0; d < 9; d++) {
+        if (!water_reach(a, d)) continue;
         const double dx = xs[a] * sci0[a][d];
         x[d] += dx;
         v[d] += dx * invdt;
@@ -113,19 +131,33 @@ __device__ inline void rattle_water(const double *x, double *v, const double *m,
   WaterC c;
   water_constraints(x, bond, t0, c);
   double sci0[3][9], xs[3], s[3][3];
+#pragma unroll
   for (int a = 0; a < 3; a++) {
     xs[a] = 0;
-    for (int d = 0; d < 9; d++) { sci0[a][d] = c.g[a][d] / m[d / 3]; xs[a] -= c.g[a][d] * v[d]; }
+#pragma unroll
+    for (int d = 0; d < 9; d++) {
+      sci0[a][d] = 0.0;
+      if (!water_reach(a, d)) continue;
+      sci0[a][d] = c.g[a][d] / m[d / 3];
+      xs[a] -= c.g[a][d] * v[d];
+    }
   }
+#pragma unroll
   for (int a = 0; a < 3; a++)
+#pragma unroll
     for (int b = a; b < 3; b++) {
       double t = 0;
-      for (int d = 0; d < 9; d++) t += c.g[a][d] * sci0[b][d];
+#pragma unroll
+      for (int d = 0; d < 9; d++)
+        if (water_reach(a, d) && water_reach(b, d)) t += c.g[a][d] * sci0[b][d];
       s[a][b] = s[b][a] = t;
     }
   if (!solve3(s, xs)) return;
+#pragma unroll
   for (int a = 0; a < 3; a++)
-    for (int d = 0; d < 9; d++) v[d] += xs[a] * sci0[a][d];
+#pragma unroll
+    for (int d = 0; d < 9; d++)
+      if (water_reach(a, d)) v[d] += xs[a] * sci0[a][d];
 }
 
 }  // namespace mmamd
