@@ -327,3 +327,35 @@ def test_ik_with_ions(gpu_lib):
     assert rel(f, of) <= 1e-5
     assert abs(en[0] - oen[0]) <= 1e-5 * abs(oen[3])
     assert rel(w, ow.reshape(3, 3)) <= 1e-5
+
+
+def test_ik_replica_exchange(gpu_lib):
+    """Replica exchange over a batch with one ladder under ik P3M and one
+    under real-space differentiation (three 512-water replicas each, 300 to
+    450 K): exchange intervals through the captured step graph, then every
+    system's forces against the oracle at its final positions and with its
+    own parameters."""
+    systems, params, kT, ladders = W.remd_batch(nrep=3, sizes=((8, 24.86), (8, 24.86)))
+    for k in ladders[1]:
+        params[k]["ik"] = 1
+    print("[ik] remd", [(s.nsite, p["kmode"], p["ik"]) for s, p in zip(systems, params)],
+          flush=True)
+    e = Engine(systems, params)
+    e.set_positions(np.concatenate([s.r for s in systems]))
+    e.set_velocities(np.concatenate([s.v for s in systems]))
+    e.md_setup(dt=0.001)
+    e.remd_setup(ladders, kT, seed=3)
+    e.remd_run(3, 10, reinit_velocities=True, graph=True)
+    st, cfg = e.remd_stats()
+    print("[ik] remd tries/accepts", st.tolist(), "config", cfg.tolist())
+    assert e.errors() == 0
+    assert sorted(cfg[:3]) == [0, 1, 2] and sorted(cfg[3:]) == [3, 4, 5]
+    e.compute()
+    r, f, off = e.positions(), e.forces(), e.offsets
+    for k, (s, p) in enumerate(zip(systems, params)):
+        t = W.System(**{a: getattr(s, a) for a in s.__dataclass_fields__})
+        t.r = np.ascontiguousarray(r[off[k]:off[k + 1]])
+        of, _ = O.evaluate(t, p)
+        err = rel(f[off[k]:off[k + 1]], of)
+        print(f"[ik] remd system {k} ik={p['ik']}: force rel {err:.3e}")
+        assert err <= 1e-5, k
