@@ -372,7 +372,8 @@ int mm_tune_kspace(double boxl, int nsite, double rc, double tol, int allow_ewal
  * R . dF/dq (R = 2 (F - f_ref)) and, per LJ type pair, the derivatives of E
  * and L with respect to a = 4 e s^12 and b = 4 e s^6; the reciprocal-space
  * adjoint (Ewald adjoint structure factors, or one P3M spread of the
- * adjoint source with the forward's convolution and gather); sigma and
+ * adjoint source with the forward's convolution and gather, or for P3M
+ * with ik differentiation three forward ik passes); sigma and
  * epsilon through the mixing rule on the host.  Pair arithmetic FP32,
  * accumulation FP64.  Outputs (any may be NULL except ntype): ntype;
  * site_type[nsite]; type_params[3*ntype]; dE, dL [3*ntype] ordered

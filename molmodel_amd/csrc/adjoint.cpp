@@ -301,11 +301,12 @@ int ensure(T **p, size_t n) {
 
 // Gradients with every pair term recomputed by the adjoint kernels
 // (adjoint.hip): one pass over the neighbour rows for the real-space and
-// excluded-pair terms, the reciprocal-space adjoint (Ewald structure factors
-// or one P3M spread + convolution + gather), reduced per site type on the
-// host.  Same site types, outputs and chain rule through the mixing rule as
-// mm_param_gradients_basis, which evaluates a basis of forward passes
-// instead.  ms: device time of the adjoint passes (may be NULL).
+// excluded-pair terms, the reciprocal-space adjoint (Ewald structure factors,
+// one P3M spread + convolution + gather, or three forward ik passes),
+// reduced per site type on the host.  Same site types, outputs and chain
+// rule through the mixing rule as mm_param_gradients_basis, which evaluates
+// a basis of forward passes instead.  ms: device time of the adjoint passes
+// (may be NULL).
 extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, int maxtypes,
                                   int *ntype_out, int *site_type, double *type_params,
                                   double *dE, double *dL, double *loss, double *ms) {
@@ -317,11 +318,6 @@ extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, in
   const SystemInput &S = e.in[sys];
   const SysParams &P = e.hsp[sys];
   const int N = S.nsite, NT = e.nsite;
-  if (P.kmode == 1 && P.ik) {
-    set_last_error("mm_param_gradients: the P3M adjoint is built for real-space differentiation; "
-                   "use mm_param_gradients_basis for a system with ik");
-    return MM_ERR_ARG;
-  }
   // site types: distinct (q, sigma, epsilon, mask), in order of appearance
   std::map<std::tuple<double, double, double, int>, int> tmap;
   std::vector<int> ty(N);
@@ -399,6 +395,7 @@ extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, in
   TRY_ADJ(ensure(&e.d_adjkres, 4 * (size_t)NT + 16));
   TRY_ADJ(ensure(&e.d_adjacc, (size_t)NACC));
   TRY_ADJ(ensure(&e.d_adjsrc, (size_t)NT));
+  TRY_ADJ(ensure(&e.d_adjq, (size_t)NT));
   MM_HIP(hipMemcpy(e.d_adjR, Rsoa.data(), sizeof(double) * Rsoa.size(), hipMemcpyHostToDevice));
   // P3M: the adjoint source is scaled to at most 1 per site so that the
   // spreading's fixed-point sums keep the forward's range
@@ -407,8 +404,10 @@ extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, in
     double mx = 0;
     for (int i = 0; i < N; i++)
       for (int d = 0; d < 3; d++) {
-        double v = 0;
-        for (int c = 0; c < 3; c++) v += R[3 * i + c] * P.T[3 * c + d];
+        // ik: the source components are q_i R_i; otherwise q_i (R_i T)
+        double v = P.ik ? R[3 * i + d] : 0.0;
+        if (!P.ik)
+          for (int c = 0; c < 3; c++) v += R[3 * i + c] * P.T[3 * c + d];
         mx = std::max(mx, fabs(S.q[i] * v));
       }
     scale = mx > 0 ? 1.0 / mx : 1.0;
@@ -420,7 +419,9 @@ extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, in
   int rc = launch_adj_pair(e, sys, e.d_adjR, e.d_chi, e.d_adjpart);
   if (!rc && f_ref && P.kmode == 0)
     rc = launch_adj_ewald(e, sys, e.d_adjR, e.d_adjksc, e.d_adjkres, e.d_adjacc);
-  if (!rc && f_ref && P.kmode == 1) {
+  if (!rc && f_ref && P.kmode == 1 && P.ik)
+    rc = launch_adj_p3m_ik(e, sys, e.d_adjR, scale, e.d_adjq, e.d_adjkres);
+  if (!rc && f_ref && P.kmode == 1 && !P.ik) {
     rc = launch_adj_p3m_src(e, sys, e.d_adjR, scale, e.d_adjsrc);
     if (!rc) rc = launch_p3m_adjoint(e, sys, e.d_adjsrc, e.stream);
   }
@@ -438,10 +439,10 @@ extern "C" int mm_param_gradients(mm_engine *m, int sys, const double *f_ref, in
   std::vector<double> chi(N), part((size_t)nblk * nslot), psi(4 * (size_t)N, 0.0);
   MM_HIP(hipMemcpy(chi.data(), e.d_chi + P.site0, sizeof(double) * N, hipMemcpyDeviceToHost));
   MM_HIP(hipMemcpy(part.data(), e.d_adjpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
-  if (f_ref && P.kmode == 0)
+  if (f_ref && (P.kmode == 0 || (P.kmode == 1 && P.ik)))
     MM_HIP(hipMemcpy(psi.data(), e.d_adjkres + 4L * P.site0, sizeof(double) * psi.size(),
                      hipMemcpyDeviceToHost));
-  if (f_ref && P.kmode == 1)
+  if (f_ref && P.kmode == 1 && !P.ik)
     MM_HIP(hipMemcpy(psi.data(), e.d_kres + 4L * P.site0, sizeof(double) * psi.size(),
                      hipMemcpyDeviceToHost));
   {

@@ -28,7 +28,8 @@
 // B_k = sum_i q_i (R_i.k) cos(k.r_i), psi_j = sum_k 2 a_k (A_k cos - B_k sin)
 // (k_ks_field with (S, C) -> (-B, A)); for P3M by spreading the adjoint
 // source q_i (R_i T) with derivative weights, the same convolution and an
-// ordinary gather (pppm.hip, launch_p3m_adjoint).
+// ordinary gather (pppm.hip, launch_p3m_adjoint); for P3M with the field
+// formed on the mesh (ik) by three forward ik passes (launch_adj_p3m_ik).
 // Pair arithmetic is FP32 with exact erfc/exp (the forward's table stands
 // for the same function), accumulation FP64.
 #include "engine.hpp"
@@ -242,6 +243,29 @@ __global__ void k_adj_p3m_src(const SysParams *__restrict__ sp, const double *__
                         (float)(qq * (Rx * T[2] + Ry * T[5] + Rz * T[8])), 0.f);
 }
 
+// ik P3M adjoint, component c: the charges of one forward ik pass,
+// qsrc_i = scale q_i R_i,c
+__global__ void k_adj_ik_src(const SysParams *__restrict__ sp, const double *__restrict__ q,
+                             const double *__restrict__ R, int N, double scale, int c,
+                             double *__restrict__ qsrc) {
+  const SysParams &P = sp[0];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nsite) return;
+  const int si = P.site0 + i;
+  qsrc[si] = scale * q[si] * R[(long)c * N + si];
+}
+
+// psi_i = -sum_c (field component c of pass c at site i), summed in c order
+__global__ void k_adj_ik_acc(const SysParams *__restrict__ sp, const double *__restrict__ kres,
+                             int c, double *__restrict__ psi) {
+  const SysParams &P = sp[0];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nsite) return;
+  const int si = P.site0 + i;
+  const double v = -kres[4L * si + 1 + c];
+  psi[4L * si] = c == 0 ? v : psi[4L * si] + v;
+}
+
 int adj_part_slots() { return ADJ_MAXT * ADJ_MAXT * 4; }
 int adj_blocks(const Engine &e, int sys) { return (e.hsp[sys].ngroup + ADJ_TB - 1) / ADJ_TB; }
 
@@ -266,6 +290,26 @@ int launch_adj_ewald(Engine &e, int sys, const double *R, double *ksc_adj, doubl
   hipLaunchKernelGGL(k_adj_ks_sf, dim3(P.nk), dim3(256), 0, e.stream, e.d_sp + sys,
                      (const double4 *)e.d_kvec, e.d_r, e.d_q, R, e.nsite, ksc_adj);
   return launch_kspace_field_only(e, sys, ksc_adj, kres_adj, acc_scratch);
+}
+
+// psi of an ik P3M system into psi[4 i]: F_i = q_i E_i with E_a = W^T M_a W q
+// and M_a^T = -M_a (M_a: the mesh operator of multiplier -i k_a G, odd), so
+// psi_j = sum_i q_i R_i . dE_i/dq_j = -sum_a [E of a forward pass whose
+// charges are q_i R_i,a]_a at site j: three forward ik passes
+int launch_adj_p3m_ik(Engine &e, int sys, const double *R, double scale, double *qsrc,
+                      double *psi) {
+  const SysParams &P = e.hsp[sys];
+  if (P.kmode != 1 || !P.ik) return 0;
+  const dim3 g((P.nsite + 255) / 256);
+  for (int c = 0; c < 3; c++) {
+    hipLaunchKernelGGL(k_adj_ik_src, g, dim3(256), 0, e.stream, e.d_sp + sys, e.d_q, R,
+                       e.nsite, scale, c, qsrc);
+    const int rc = launch_p3m_ik_pass(e, sys, qsrc, e.stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_adj_ik_acc, g, dim3(256), 0, e.stream, e.d_sp + sys, e.d_kres, c, psi);
+  }
+  MM_HIP(hipGetLastError());
+  return 0;
 }
 
 int launch_adj_p3m_src(Engine &e, int sys, const double *R, double scale, float4 *src) {

@@ -975,7 +975,7 @@ void engine_destroy(Engine &e) {
                   e.d_nlb, e.d_ladder, e.d_kT, e.d_tail, e.d_rtmp, e.d_mom, e.d_perm,
                   e.d_remd_stats, e.d_tile_start, e.d_snap, e.d_snap_perm, e.d_snap_stats,
                   e.d_adjR, e.d_chi, e.d_adjpart, e.d_adjksc, e.d_adjkres, e.d_adjacc,
-                  e.d_adjsrc};
+                  e.d_adjsrc, e.d_adjq};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (e.stream) hipStreamDestroy(e.stream);

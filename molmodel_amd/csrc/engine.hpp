@@ -171,6 +171,7 @@ struct Engine {
   double *d_adjR = nullptr, *d_chi = nullptr, *d_adjpart = nullptr, *d_adjksc = nullptr;
   double *d_adjkres = nullptr, *d_adjacc = nullptr;
   float4 *d_adjsrc = nullptr;
+  double *d_adjq = nullptr;
   // state saved at the start of an MD block or exchange interval, restored
   // when an overflow makes the block run again (r, v: 6 N; exchange
   // permutation and statistics)
@@ -257,12 +258,17 @@ int launch_kspace_raw(const SysParams *sp, const double4 *kvec, int nk, int nsit
 int launch_kspace_field_only(Engine &e, int sys, const double *ksc, double *kres_out,
                              double *acc_out);
 int launch_p3m_adjoint(Engine &e, int sys, const float4 *src, hipStream_t st);
+// one forward ik P3M pass of system sys with per-site charges q (global
+// site index): bin, spread, the ik convolution, gather into kres
+int launch_p3m_ik_pass(Engine &e, int sys, const double *q, hipStream_t st);
 int adj_part_slots();
 int adj_blocks(const Engine &e, int sys);
 int launch_adj_pair(Engine &e, int sys, const double *R, double *chi, double *part);
 int launch_adj_ewald(Engine &e, int sys, const double *R, double *ksc_adj, double *kres_adj,
                      double *acc_scratch);
 int launch_adj_p3m_src(Engine &e, int sys, const double *R, double scale, float4 *src);
+int launch_adj_p3m_ik(Engine &e, int sys, const double *R, double scale, double *qsrc,
+                      double *psi);
 int launch_finish_ex(Engine &e, int mode, int want_virial, double *phi_out);
 int launch_field_out(Engine &e, double *out, double *ulj, int min_image = 1);
 int launch_skin(Engine &e);

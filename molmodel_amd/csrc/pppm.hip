@@ -911,4 +911,54 @@ int launch_p3m_adjoint(Engine &e, int sys, const float4 *src, hipStream_t st) {
   return 0;
 }
 
+// One forward ik pass of one system with the charges q (global site index)
+// in place of its own: bin, spread by pulling, the ik convolution (no
+// energy) and the ik gather, which leaves (phi, E) of the pass in kres[4 i]
+// of the system's sites.  The ik field operator is a real circulant whose
+// multiplier -i k_a G(k) is odd, so its transpose is its negative; the
+// adjoint of the ik force therefore needs nothing but forward passes.
+int launch_p3m_ik_pass(Engine &e, int sys, const double *q, hipStream_t st) {
+  const SysParams &P = e.hsp[sys];
+  if (P.kmode != 1 || !P.ik) return 0;
+  const SysParams *sp = e.d_sp + sys;
+  const int tb = 256;
+  const dim3 gs((P.nsite + tb - 1) / tb, 1);
+  hipLaunchKernelGGL(k_p3m_bin, gs, dim3(tb), 0, st, sp, e.d_r, q, e.nsite, e.d_sgrid,
+                     e.d_sfrac, e.d_bin_fill, e.bcap, e.d_bgrid, e.d_bfrac, e.d_err,
+                     e.d_bin_fill + e.nbin_total);
+  const dim3 gb(P.nbx * P.nby, 1);
+#define IK_SPREAD(PO)                                                                        \
+  case PO:                                                                                   \
+    hipLaunchKernelGGL((k_p3m_spread<PO, false>), gb, dim3(256), 0, st, sp, e.d_bin_fill,    \
+                       e.bcap, e.d_bgrid, e.d_bfrac, e.d_mesh, nullptr);                     \
+    break;
+  switch (P.order) {
+    IK_SPREAD(1) IK_SPREAD(2) IK_SPREAD(3) IK_SPREAD(4) IK_SPREAD(5) IK_SPREAD(6) IK_SPREAD(7)
+    default: set_last_error("assignment order out of range"); return -1;
+  }
+#undef IK_SPREAD
+  const int nzh = P.nz / 2 + 1;
+  const int maxz = (P.nx * P.ny + FFT_LINES - 1) / FFT_LINES;
+  const int maxy = (P.nx * nzh + FFT_LINES - 1) / FFT_LINES;
+  const int maxx = (P.ny * nzh + FFT_LINES - 1) / FFT_LINES;
+  bool pow2 = true;
+  for (auto &p : e.hsp)   // the engine's one FFT build, as in the forward
+    if (p.kmode == 1)
+      for (int n : {p.nx, p.ny, p.nz}) pow2 &= (n & (n - 1)) == 0;
+  if (pow2) launch_fft<true>(e, 0, 0, st, maxx, maxy, maxz, sp, 1, true, nullptr);
+  else launch_fft<false>(e, 0, 0, st, maxx, maxy, maxz, sp, 1, true, nullptr);
+#define IK_GATHER(PO)                                                                       \
+  case PO:                                                                                  \
+    hipLaunchKernelGGL((k_p3m_gather<PO, true>), gs, dim3(tb), 0, st, sp, e.d_sgrid,        \
+                       e.d_sfrac, e.d_mesh, e.d_kres, e.d_bin_fill);                        \
+    break;
+  switch (P.order) {
+    IK_GATHER(1) IK_GATHER(2) IK_GATHER(3) IK_GATHER(4) IK_GATHER(5) IK_GATHER(6) IK_GATHER(7)
+    default: break;
+  }
+#undef IK_GATHER
+  MM_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace mmamd

@@ -1,5 +1,6 @@
 """Per-type parameter gradients on the water + ions box (1000 waters, 20 Na+,
-20 Cl-), with Ewald (as the benchmark definition sets it) and with P3M.
+20 Cl-), with Ewald (as the benchmark definition sets it) and with P3M
+(real-space and ik differentiation).
 
 Reference forces come from the FP64 oracle at parameters perturbed by a
 seeded 5% Gaussian; L = sum_i |F_i - F_ref_i|^2.  The gradients of the
@@ -41,12 +42,12 @@ def with_params(s, types, site_type):
 
 def params_for(kind):
     s, p = W.config("water1000_ions")
-    if kind == "p3m":
-        p = dict(p, kmode=1, mesh=(32, 32, 32), order=5)
+    if kind in ("p3m", "p3m_ik"):
+        p = dict(p, kmode=1, mesh=(32, 32, 32), order=5, ik=int(kind == "p3m_ik"))
     return s, p
 
 
-@pytest.fixture(scope="module", params=["ewald", "p3m"])
+@pytest.fixture(scope="module", params=["ewald", "p3m", "p3m_ik"])
 def setup(request, gpu_lib):
     s, p = params_for(request.param)
     e = Engine(s, p)
