@@ -129,17 +129,21 @@ def test_ik_md_then_forces(gpu_lib):
     assert rel(e.forces(), of) <= 1e-5
 
 
-def test_ik_adjoint_matches_basis(gpu_lib):
+@pytest.mark.parametrize("batch", [False, True])
+def test_ik_adjoint_matches_basis(gpu_lib, batch):
     """The adjoint of an ik system (three forward ik passes for the mesh
-    part) gives the basis method's energy and loss gradients."""
+    part) gives the basis method's energy and loss gradients, alone and as
+    the second system of a batch whose first differentiates in real space."""
     s, p = small_box(1)
-    e = Engine(s, p)
-    e.set_positions(s.r)
-    g0 = e.param_gradients(0)
+    mem = ([small_box(0, seed=32)] if batch else []) + [(s, p)]
+    e = Engine([m[0] for m in mem], [m[1] for m in mem])
+    e.set_positions(np.concatenate([m[0].r for m in mem]))
+    k = len(mem) - 1
+    g0 = e.param_gradients(k)
     f_ref, _ = O.evaluate(s, dict(p, eta=p["eta"] * 1.01))   # any nearby reference
-    g = e.param_gradients(0, f_ref=f_ref)
-    gb = e.param_gradients(0, f_ref=f_ref, basis=True)
-    print(f"[ik] adjoint {g['ms']:.3f} ms; basis_err {gb['basis_err']:.1e}")
+    g = e.param_gradients(k, f_ref=f_ref)
+    gb = e.param_gradients(k, f_ref=f_ref, basis=True)
+    print(f"[ik] system {k}: adjoint {g['ms']:.3f} ms; basis_err {gb['basis_err']:.1e}")
     assert g0["ntype"] >= 2 and gb["basis_err"] < 1e-5
     for name in ("dE", "dL"):
         scale = np.abs(gb[name]).max()
