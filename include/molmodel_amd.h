@@ -340,6 +340,17 @@ int mm_nlist_destroy(mm_nlist *nl);
 int mm_remd_setup(mm_engine *e, int nladder, const int *ladder, const double *kT,
                   unsigned long long seed);
 int mm_remd_run(mm_engine *e, int nintervals, int steps, int reinit_velocities, int use_graph);
+/* The reference's schedule (MDyn::run, mdyn.cpp:142-186, do_every at
+ * mdyn.cpp:49-52): nsteps MD steps; after each, with probability
+ * 1/reinit_interval new velocities (the constant-temperature move,
+ * reinit_velocities_interval) and, independently, with probability
+ * 1/exchange_interval an exchange attempt (replica_exchange_interval);
+ * an interval of 0 disables its move.  One schedule for the whole batch,
+ * drawn from counter-based random numbers of the setup's seed and the step
+ * count since mm_remd_setup; the redraw follows the exchange when both
+ * fall on one step.  nexchange, nreinit (may be NULL): the moves made. */
+int mm_remd_run_stochastic(mm_engine *e, int nsteps, int exchange_interval, int reinit_interval,
+                           int use_graph, long long *nexchange, long long *nreinit);
 int mm_remd_stats(mm_engine *e, long long *tries_accepts, int *config);
 
 /* Reciprocal-space parameters for one cubic system chosen by this library's

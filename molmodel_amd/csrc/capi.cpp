@@ -900,6 +900,48 @@ int mm_remd_setup(mm_engine *m, int nladder, const int *ladder, const double *kT
   e.nladder = nladder;
   e.remd_seed = seed;
   e.remd_attempt = 0;
+  e.remd_step = 0;
+  return MM_OK;
+}
+
+// steps MD steps, then the exchange attempt (if exchange) and the velocity
+// redraw (if reinit), then forces of the configurations now held; a block
+// whose neighbour rows or P3M bins overflowed runs again from its start with
+// the same random numbers
+static int remd_block(mm_engine *m, int steps, bool exchange, bool reinit, int use_graph) {
+  Engine &e = m->e;
+  for (int attempt = 0;; attempt++) {
+    TRY(save_state(e, 1));
+    if (steps > 0) TRY(md_run_async(m, steps, use_graph));
+    const long long a = e.remd_attempt;
+    // end of the block's last step: the exchange attempt (mdyn.cpp:185-186),
+    // then the velocity redraw (mdyn.cpp:163-167; the reference redraws
+    // before exchanging, which leaves swapped rigid molecules with velocities
+    // constrained for the configuration they left)
+    if (exchange) TRY(launch_remd_exchange(e, a));
+    if (reinit) TRY(launch_remd_velocities(e, a));
+    if (exchange) {
+      e.need_pack = 1;   // positions moved between systems
+      // forces and energies of the configurations now held; swapped systems
+      // rebuild their neighbour lists first
+      TRY(enqueue_forces(m, 0, 0));
+      TRY(launch_finish_ex(e, FIN_FORCES, 0, e.d_phi));
+    }
+    int f = 0;
+    TRY(read_err(e, &f));
+    if (replayable(f) && attempt < 8) {
+      TRY(clear_err(e));
+      const int g = grow_overflowed(e, f);
+      if (g < 0) return g;
+      TRY(restore_state(e, 1));
+      m->pos_dirty = 1;
+      TRY(forces_current(m));
+      continue;
+    }
+    if (f) return md_report(e, f);
+    break;
+  }
+  if (exchange || reinit) e.remd_attempt++;
   return MM_OK;
 }
 
@@ -909,39 +951,62 @@ int mm_remd_run(mm_engine *m, int nintervals, int steps, int reinit_velocities, 
   if (!e.d_ladder) { set_last_error("mm_remd_setup first"); return MM_ERR_ARG; }
   if (nintervals < 0 || steps < 0) { set_last_error("bad interval arguments"); return MM_ERR_ARG; }
   TRY(forces_current(m));
-  for (int it = 0; it < nintervals; it++) {
-    for (int attempt = 0;; attempt++) {
-      TRY(save_state(e, 1));
-      if (steps > 0) TRY(md_run_async(m, steps, use_graph));
-      const long long a = e.remd_attempt;
-      // end of the interval's last step: the exchange attempt (mdyn.cpp:185-186),
-      // then the velocity redraw (mdyn.cpp:163-167; the reference redraws
-      // before exchanging, which leaves swapped rigid molecules with velocities
-      // constrained for the configuration they left)
-      TRY(launch_remd_exchange(e, a));
-      if (reinit_velocities) TRY(launch_remd_velocities(e, a));
-      e.need_pack = 1;   // positions moved between systems
-      // forces and energies of the configurations now held; swapped systems
-      // rebuild their neighbour lists first
-      TRY(enqueue_forces(m, 0, 0));
-      TRY(launch_finish_ex(e, FIN_FORCES, 0, e.d_phi));
-      int f = 0;
-      TRY(read_err(e, &f));
-      if (replayable(f) && attempt < 8) {
-        // the interval runs again from its start with the same random numbers
-        TRY(clear_err(e));
-        const int g = grow_overflowed(e, f);
-        if (g < 0) return g;
-        TRY(restore_state(e, 1));
-        m->pos_dirty = 1;
-        TRY(forces_current(m));
-        continue;
-      }
-      if (f) return md_report(e, f);
-      break;
-    }
-    e.remd_attempt++;
+  for (int it = 0; it < nintervals; it++) TRY(remd_block(m, steps, true, reinit_velocities != 0, use_graph));
+  return MM_OK;
+}
+
+namespace {
+// splitmix64 (Steele, Lea, Flood 2014) and a uniform double in [0, 1) of
+// (seed, stream, index), the construction the device's exchange uses
+unsigned long long splitmix_host(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double uniform01_host(unsigned long long seed, unsigned long long stream, unsigned long long idx) {
+  const unsigned long long k = splitmix_host(seed * 0x100000001B3ull + stream);
+  return (double)(splitmix_host(idx ^ k) >> 11) * (1.0 / 9007199254740992.0);
+}
+constexpr unsigned long long STREAM_REINIT = 0x5343484544000001ull;   // schedule draws
+constexpr unsigned long long STREAM_EXCHANGE = 0x5343484544000002ull;
+}  // namespace
+
+// do_every (mdyn.cpp:49-52): after each step, a move happens if
+// interval * u < 1 for a fresh uniform u; the velocity redraw
+// (mdyn.cpp:163-167) and the exchange attempt (mdyn.cpp:185-186) draw
+// independently.  The draws are counter-based in the engine's step count,
+// one schedule for all systems of the batch.
+int mm_remd_run_stochastic(mm_engine *m, int nsteps, int exchange_interval, int reinit_interval,
+                           int use_graph, long long *nexchange, long long *nreinit) {
+  CHECK_ENGINE(m);
+  Engine &e = m->e;
+  if (!e.d_ladder) { set_last_error("mm_remd_setup first"); return MM_ERR_ARG; }
+  if (nsteps < 0 || exchange_interval < 0 || reinit_interval < 0) {
+    set_last_error("bad schedule arguments");
+    return MM_ERR_ARG;
   }
+  TRY(forces_current(m));
+  long long nx = 0, nr = 0;
+  int done = 0;
+  while (done < nsteps) {
+    int s = done;
+    bool ex = false, rv = false;
+    while (s < nsteps && !ex && !rv) {
+      const unsigned long long k = (unsigned long long)(e.remd_step + s);
+      rv = reinit_interval > 0 && reinit_interval * uniform01_host(e.remd_seed, STREAM_REINIT, k) < 1;
+      ex = exchange_interval > 0 &&
+           exchange_interval * uniform01_host(e.remd_seed, STREAM_EXCHANGE, k) < 1;
+      s++;
+    }
+    TRY(remd_block(m, s - done, ex, rv, use_graph));
+    nx += ex;
+    nr += rv;
+    done = s;
+  }
+  e.remd_step += nsteps;
+  if (nexchange) *nexchange = nx;
+  if (nreinit) *nreinit = nr;
   return MM_OK;
 }
 

@@ -163,6 +163,7 @@ struct Engine {
   long long *d_remd_stats = nullptr;
   unsigned long long remd_seed = 0;
   long long remd_attempt = 0;
+  long long remd_step = 0;   // MD steps run by the stochastic schedule
   // per system, the two parameter sets of a perturbation (ElecLJ's
   // types_changed(pert), eleclj.cpp:273-308): q, sigma, epsilon of state A
   // then of state B (6 nsite), empty when none is set

@@ -14,9 +14,10 @@
 // momentum.  Everything runs on the device, so a run of exchange intervals
 // needs no host synchronisation.
 //
-// Differences from the reference, by design: attempts and velocity redraws
-// happen at fixed intervals instead of with probability 1/interval per step
-// (do_every, mdyn.cpp:49-52); the redraw follows the exchange, so swapped
+// Differences from the reference, by design: with mm_remd_run, attempts and
+// velocity redraws happen at fixed intervals instead of with probability
+// 1/interval per step (do_every, mdyn.cpp:49-52; mm_remd_run_stochastic
+// keeps those draws); the redraw follows the exchange, so swapped
 // rigid molecules get velocities constrained for the configuration they
 // now hold; forces are recomputed after a swap (the
 // reference half-kicks once with the forces of the configuration it held

@@ -66,6 +66,7 @@ def lib():
         L.mm_engine_sizes.argtypes = [_P, _P, _P]
         L.mm_remd_setup.argtypes = [_P, _I, _P, _P, C.c_ulonglong]
         L.mm_remd_run.argtypes = [_P, _I, _I, _I, _I]
+        L.mm_remd_run_stochastic.argtypes = [_P, _I, _I, _I, _I, _P, _P]
         L.mm_remd_stats.argtypes = [_P, _P, _P]
         L.mm_tune_kspace.argtypes = [_D, _I, _D, _D, _I, _P]
         L.mm_param_gradients.argtypes = [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
@@ -517,6 +518,15 @@ class Engine:
     def remd_run(self, nintervals, steps, reinit_velocities=True, graph=True):
         _check(lib().mm_remd_run(self.h, int(nintervals), int(steps),
                                  1 if reinit_velocities else 0, 1 if graph else 0))
+
+    def remd_run_stochastic(self, nsteps, exchange_interval, reinit_interval=0, graph=True):
+        """nsteps MD steps with the reference's per-step move draws
+        (do_every, mdyn.cpp:49-52); returns (exchanges, velocity redraws)."""
+        nx, nr = C.c_longlong(0), C.c_longlong(0)
+        _check(lib().mm_remd_run_stochastic(self.h, int(nsteps), int(exchange_interval),
+                                            int(reinit_interval), 1 if graph else 0,
+                                            C.byref(nx), C.byref(nr)))
+        return nx.value, nr.value
 
     def remd_stats(self):
         st = np.zeros((self.nsys, 2), dtype=np.int64)

@@ -152,3 +152,37 @@ def test_mixed_batch_intervals(gpu_lib):
     assert st[0, 0] == 3 and st[2, 0] == 0
     assert np.all(np.isfinite(en[:, :5]))
     assert e.nl_builds() > b0
+
+
+STREAM_REINIT = 0x5343484544000001
+STREAM_EXCHANGE = 0x5343484544000002
+
+
+def schedule(seed, first, nsteps, xint, rint):
+    """do_every (mdyn.cpp:49-52) per step, restated: (exchanges, redraws)."""
+    nx = nr = 0
+    for k in range(first, first + nsteps):
+        nr += rint > 0 and rint * uniform01(seed, STREAM_REINIT, k) < 1
+        nx += xint > 0 and xint * uniform01(seed, STREAM_EXCHANGE, k) < 1
+    return nx, nr
+
+
+def test_stochastic_schedule(gpu_lib):
+    """The reference's per-step move draws: the engine makes exactly the
+    exchanges and velocity redraws a host restatement of the schedule
+    predicts, continues the step count across calls, and every exchange
+    attempt is counted in the ladders' tries."""
+    systems, params, kT, ladders = W.remd_batch(nrep=3, sizes=((6, 18.64), (8, 24.86)))
+    e = make_engine(systems, params)
+    seed = 9
+    e.remd_setup(ladders, kT, seed=seed)
+    got = [e.remd_run_stochastic(40, 7, 13), e.remd_run_stochastic(25, 5, 0)]
+    want = [schedule(seed, 0, 40, 7, 13), schedule(seed, 40, 25, 5, 0)]
+    print("[remd] stochastic moves", got, "host schedule", want, flush=True)
+    assert got == want and want[0][0] > 0 and want[0][1] > 0
+    st, cfg = e.remd_stats()
+    assert st[0, 0] == got[0][0] + got[1][0] and st[3, 0] == st[0, 0]
+    assert sorted(cfg[:3]) == [0, 1, 2] and sorted(cfg[3:]) == [3, 4, 5]
+    assert e.errors() == 0 and np.all(np.isfinite(e.energies()[:, :5]))
+    # interval 1: an attempt after every step
+    assert e.remd_run_stochastic(3, 1, 0) == (3, 0)
