@@ -97,3 +97,41 @@ def test_deck_and_restart_drive_the_engine(gpu_lib, tmp_path):
     assert ferr <= 1e-5
     assert abs(en1[0] - oen[0]) <= 1e-5 * abs(oen[3])
     assert abs(en1[1] - oen[1]) <= 1e-5 * abs(oen[3])
+
+
+POTENTIAL_P3M_IK = """{
+  rspace_cutoff 10  smoothing_width 1  neighbor_list_width 2
+  use_p3m yes  p3m_ik_differentiate yes  p3m_grid_spacing 1.0
+  p3m_assignment_order 5  ewald_screening 0.32
+  fixed_charge { OW -0.8476 HW 0.4238 }
+  sigma { OW 3.166 }  epsilon { OW 0.1553 HW 0 }
+}
+"""
+
+
+def test_p3m_ik_deck_drives_the_engine(gpu_lib, tmp_path):
+    """An ElecLJ block with use_p3m and p3m_ik_differentiate (eleclj.cpp:
+    12-40, 201-210) configures an engine whose forces and energies match
+    the oracle's ik P3M at the descriptor's parameters."""
+    s, _ = W.config("water12k_p3m")
+    pot = os.path.join(str(tmp_path), "pot.inp")
+    with open(pot, "w") as f:
+        f.write(POTENTIAL_P3M_IK)
+    m = E.MsimElecLJ(E.msim_preprocess([pot]))
+    prm = m.desc(s.boxl)
+    print(f"[msim] ik deck: {prm}", flush=True)
+    assert prm["kmode"] == 1 and prm["ik"] == 1 and prm["order"] == 5 and prm["eta"] == 0.32
+    typ = ["OW", "HW", "HW"] * (s.nsite // 3)
+    q, sig, eps, mask = m.site_params(typ)
+    s2 = dataclasses.replace(s, q=q, sig=sig, eps=eps, site_mask=mask)
+    params = {k: prm[k] for k in ("rc", "smoothing_width", "skin", "eta", "kmode", "kc", "mesh",
+                                  "order", "nalias", "ik", "include_lj_correction")}
+    e = E.Engine(s2, params)
+    e.set_positions(s2.r)
+    e.compute()
+    f, en = e.forces(), e.energies()[0]
+    of, oen = O.evaluate(s2, params)
+    ferr = float(np.linalg.norm(f - of) / np.linalg.norm(of))
+    print(f"[msim] ik deck: mesh {prm['mesh']}, force rel {ferr:.3e}", flush=True)
+    assert ferr <= 1e-5
+    assert abs(en[0] - oen[0]) <= 1e-5 * abs(oen[3])
