@@ -540,6 +540,7 @@ int engine_layout(Engine &e, const std::vector<SystemInput> &in, HostLayout &h) 
       P.T[2] *= S.nz; P.T[5] *= S.nz; P.T[8] *= S.nz;
       const int nzh = S.nz / 2 + 1;
       const int ncomp = S.ik ? 4 : 1;   // phi [, Ex, Ey, Ez]
+      mesh0 = (mesh0 + 3) & ~3L;   // 16-byte aligned: an ik system's points are float4
       P.mesh0 = mesh0; mesh0 += ncomp * (long)S.nx * S.ny * S.nz;
       P.cmesh0 = cmesh0; cmesh0 += ncomp * (long)S.nx * S.ny * nzh;
       P.green0 = green0;

@@ -64,9 +64,11 @@ struct SysParams {
                                // halo0 + k (bx+order-1)(by+order-1) nz
   int twid0;                   // first of this system's 3 x 128 FFT twiddles
   int water;                   // every group is a three-site water (mask bytes WATER_MASKS)
-  int ik;                      // P3M field by -i k on the mesh: four real meshes (phi, Ex, Ey,
-                               // Ez) at mesh0 + c nx ny nz and four half spectra at
-                               // cmesh0 + c nx ny (nz/2+1), c = 0..3
+  int ik;                      // P3M field by -i k on the mesh: four half spectra at
+                               // cmesh0 + c nx ny (nz/2+1), c = 0..3 (phi, Ex, Ey, Ez), and
+                               // 4 nx ny nz floats of real mesh from mesh0 (the density in
+                               // the first nx ny nz, the transformed field interleaved as
+                               // (phi, Ex, Ey, Ez) per point); mesh0 is a multiple of 4
   // dynamics
   double dt;
   double bg_phi;               // neutralising background potential (or 0)

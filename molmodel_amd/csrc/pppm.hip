@@ -532,8 +532,13 @@ __global__ void __launch_bounds__(FFT_TB, 4) k_fft_pass(const SysParams *__restr
     float2 *dst = cmesh + cmesh0 + (long)line * nzh;
     for (int t = tid; t < nzh; t += FFT_T) dst[t] = rl[t];
   } else if (pass == 4) {
-    float *dst = mesh + mesh0 + (long)line * nz;
-    for (int t = tid; t < n; t += FFT_T) dst[t] = rl[t].x;
+    if (P.ik) {   // (phi, Ex, Ey, Ez) interleaved per point: the gather reads one float4
+      float *dst = mesh + P.mesh0 + 4 * ((long)line * nz) + comp;
+      for (int t = tid; t < n; t += FFT_T) dst[4 * t] = rl[t].x;
+    } else {
+      float *dst = mesh + mesh0 + (long)line * nz;
+      for (int t = tid; t < n; t += FFT_T) dst[t] = rl[t].x;
+    }
   } else {
     const int xi = line / nzh, kz = line % nzh;
     float2 *dst = cmesh + cmesh0 + (long)xi * ny * nzh + kz;
@@ -700,20 +705,22 @@ __global__ void __launch_bounds__(256) k_p3m_gather(const SysParams *__restrict_
   const float *m0 = mesh + Q.mesh0;
   float ph = 0.f, e0 = 0.f, e1 = 0.f, e2 = 0.f;
   if constexpr (IK) {
-    const long M = (long)nx * ny * nz;
+    // the last inverse pass leaves (phi, Ex, Ey, Ez) interleaved per point
+    const float4 *m4 = reinterpret_cast<const float4 *>(m0);
 #pragma unroll
     for (int a = 0; a < P; a++) {
 #pragma unroll
       for (int b = 0; b < P; b++) {
-        const float *col = m0 + ((long)ix[a] * ny + iy[b]) * nz;
+        const float4 *col = m4 + ((long)ix[a] * ny + iy[b]) * nz;
         const float wxy = wx[a] * wy[b];
 #pragma unroll
         for (int c = 0; c < P; c++) {
           const float w = wxy * wz[c];
-          ph += w * col[iz[c]];
-          e0 += w * col[M + iz[c]];
-          e1 += w * col[2 * M + iz[c]];
-          e2 += w * col[3 * M + iz[c]];
+          const float4 v = col[iz[c]];
+          ph += w * v.x;
+          e0 += w * v.y;
+          e1 += w * v.z;
+          e2 += w * v.w;
         }
       }
     }

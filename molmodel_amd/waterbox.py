@@ -254,6 +254,7 @@ def config(name: str, seed: int = 2024) -> tuple[System, dict]:
     water648_ewald     216 waters, L 18.64, Ewald eta 0.35, kc 7, rc 9
     water12k_p3m       4096 waters, L 49.7, P3M 48^3 order 5, eta 0.32, rc 10
     water98k_p3m       32768 waters, L 99.4, P3M 64^3 order 5, eta 0.32, rc 10
+    water98k_p3m_ik    the same with the P3M field formed as -i k phi on the mesh
     water1000_ions     1000 waters + 20 Na + 20 Cl, L 31.4, Ewald (eta 0.32, kc 7), rc 10
     """
     if name == "water648_ewald":
@@ -262,9 +263,10 @@ def config(name: str, seed: int = 2024) -> tuple[System, dict]:
     elif name == "water12k_p3m":
         s = water_box(16, 49.7, seed)
         p = dict(rc=10.0, smoothing_width=1.0, eta=0.32, kmode=1, mesh=(48, 48, 48), order=5)
-    elif name == "water98k_p3m":
+    elif name in ("water98k_p3m", "water98k_p3m_ik"):
         s = water_box(32, 99.4, seed)
-        p = dict(rc=10.0, smoothing_width=1.0, eta=0.32, kmode=1, mesh=(64, 64, 64), order=5)
+        p = dict(rc=10.0, smoothing_width=1.0, eta=0.32, kmode=1, mesh=(64, 64, 64), order=5,
+                 ik=int(name.endswith("_ik")))
     elif name == "water1000_ions":
         s = water_box(10, 31.4, seed, ions=(20, 20))
         p = dict(rc=10.0, smoothing_width=1.0, eta=0.32, kmode=0, kc=7.0)
